@@ -1,0 +1,234 @@
+"""ctypes binding of the CPU oracle (oracle.c) and the reference pyJac build (_ref/).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, always as the checker, never as the thing measured
+or shipped. The product package (hpc_ode_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from . import mech as _mech
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+PYJAC_PATH = os.path.join(HERE, "_ref", "libpyjac_gri30.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrMech(C.Structure):
+    _fields_ = [
+        ("K", C.c_int), ("R", C.c_int),
+        ("W", _dp), ("tmid", _dp), ("nasa", _dp),
+        ("kind", _ip), ("rev", _ip), ("nreac", _ip), ("nprod", _ip),
+        ("reac_sp", _ip), ("prod_sp", _ip), ("reac_nu", _dp), ("prod_nu", _dp),
+        ("arr", _dp), ("troe", _dp), ("eff_ptr", _ip), ("eff_sp", _ip),
+        ("eff_val", _dp), ("eff_base", _dp),
+    ]
+
+
+class OrOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("maxsteps", C.c_int),
+                ("lu_fp32", C.c_int), ("refine_sweeps", C.c_int)]
+
+
+class OrStats(C.Structure):
+    _fields_ = [("nst", C.c_long), ("nfe", C.c_long), ("nje", C.c_long),
+                ("nsetups", C.c_long), ("nni", C.c_long), ("ncfn", C.c_long),
+                ("netf", C.c_long), ("nlu", C.c_long), ("nsolve", C.c_long),
+                ("qlast", C.c_int), ("hlast", C.c_double)]
+
+
+STATS_FIELDS = ("nst", "nfe", "nje", "nsetups", "nni", "ncfn", "netf", "nlu", "nsolve")
+
+RHS_FN = C.CFUNCTYPE(C.c_int, C.c_double, _dp, _dp, C.c_void_p)
+JAC_FN = C.CFUNCTYPE(C.c_int, C.c_double, _dp, _dp, _dp, C.c_void_p)
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so (and the pyJac reference library when its sources exist)."""
+    cmd = ["make", "-C", HERE, "-j8", "all"]
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL if quiet else None)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or_rhs.argtypes = [C.POINTER(OrMech), C.c_double, _dp, _dp]
+        L.or_jac.argtypes = [C.POINTER(OrMech), C.c_double, _dp, _dp]
+        L.or_rates.argtypes = [C.POINTER(OrMech), C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.or_getrf.argtypes = [C.c_long, _dp, C.POINTER(C.c_long)]
+        L.or_getrf.restype = C.c_long
+        L.or_getrs.argtypes = [C.c_long, _dp, C.POINTER(C.c_long), _dp]
+        L.or_cvode_generic.argtypes = [C.c_int, RHS_FN, JAC_FN, C.c_void_p, C.c_double,
+                                       C.c_double, _dp, C.POINTER(OrOpts), C.POINTER(OrStats)]
+        L.or_cvode_cell.argtypes = [C.POINTER(OrMech), C.c_double, C.c_double, _dp,
+                                    C.POINTER(OrOpts), C.POINTER(OrStats)]
+        L.or_cvode_batch.argtypes = [C.POINTER(OrMech), C.c_long, _dp, _dp, _dp,
+                                     C.POINTER(OrOpts), _ip, C.POINTER(OrStats), C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """One mechanism bound to the C oracle."""
+
+    def __init__(self, path: str):
+        self.path = path
+        self.mech = _mech.load(path)
+        t = _mech.tables(self.mech)
+        self.t = t
+        keep = dict(
+            W=np.ascontiguousarray(t.W), tmid=np.ascontiguousarray(t.tmid),
+            nasa=np.ascontiguousarray(t.nasa), kind=t.kind, rev=t.rev, nreac=t.nreac,
+            nprod=t.nprod, reac_sp=np.ascontiguousarray(t.reac_sp),
+            prod_sp=np.ascontiguousarray(t.prod_sp), reac_nu=np.ascontiguousarray(t.reac_nu),
+            prod_nu=np.ascontiguousarray(t.prod_nu), arr=np.ascontiguousarray(t.arr),
+            troe=np.ascontiguousarray(t.troe), eff_ptr=t.eff_ptr,
+            eff_sp=t.eff_sp if len(t.eff_sp) else np.zeros(1, np.int32),
+            eff_val=t.eff_val if len(t.eff_val) else np.zeros(1),
+            eff_base=np.ascontiguousarray(t.eff_base))
+        self._keep = keep
+        s = OrMech()
+        s.K, s.R = t.K, t.R
+        for name in ("W", "tmid", "nasa", "reac_nu", "prod_nu", "arr", "troe", "eff_val", "eff_base"):
+            setattr(s, name, _p(keep[name]))
+        for name in ("kind", "rev", "nreac", "nprod", "reac_sp", "prod_sp", "eff_ptr", "eff_sp"):
+            setattr(s, name, _p(keep[name], _ip))
+        self.s = s
+        self.K = t.K
+        self.R = t.R
+        self.n = t.K
+
+    # --- chemistry -------------------------------------------------------
+    def rhs(self, P: float, y: np.ndarray) -> np.ndarray:
+        y = np.ascontiguousarray(y, np.float64)
+        dy = np.zeros(self.n)
+        lib().or_rhs(C.byref(self.s), P, _p(y), _p(dy))
+        return dy
+
+    def jac(self, P: float, y: np.ndarray) -> np.ndarray:
+        """Row-major J[i, j] = d dy_i / d y_j."""
+        y = np.ascontiguousarray(y, np.float64)
+        J = np.zeros(self.n * self.n)
+        lib().or_jac(C.byref(self.s), P, _p(y), _p(J))
+        return J.reshape(self.n, self.n).T.copy()
+
+    def rates(self, P: float, y: np.ndarray):
+        y = np.ascontiguousarray(y, np.float64)
+        conc = np.zeros(self.K)
+        fwd = np.zeros(self.R)
+        rev = np.zeros(self.R)
+        pm = np.zeros(self.R)
+        wdot = np.zeros(self.K)
+        lib().or_rates(C.byref(self.s), P, _p(y), _p(conc), _p(fwd), _p(rev), _p(pm), _p(wdot))
+        return dict(conc=conc, fwd=fwd, rev=rev, presmod=pm, wdot=wdot)
+
+    # --- integration -----------------------------------------------------
+    def integrate(self, P, dt, y, rtol=1e-6, atol=1e-10, maxsteps=10000, lu_fp32=False,
+                  refine_sweeps=0, nthreads=0):
+        """Integrates every row of y (shape [ncells, K]) from 0 to dt. Returns (y, status, stats)."""
+        y = np.array(y, dtype=np.float64, order="C", copy=True)
+        if y.ndim == 1:
+            y = y[None, :]
+        nc = y.shape[0]
+        P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (nc,)))
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (nc,)))
+        o = OrOpts(rtol, atol, maxsteps, int(lu_fp32), refine_sweeps)
+        status = np.zeros(nc, np.int32)
+        stats = (OrStats * nc)()
+        lib().or_cvode_batch(C.byref(self.s), nc, _p(P), _p(dt), _p(y), C.byref(o),
+                             _p(status, _ip), stats, nthreads)
+        st = {f: np.array([getattr(s, f) for s in stats]) for f in STATS_FIELDS}
+        return y, status, st
+
+
+def getrf(a: np.ndarray):
+    """SUNDIALS-style LU of a row-major square matrix; returns (lu colmajor, piv, ier)."""
+    n = a.shape[0]
+    cm = np.asfortranarray(a, dtype=np.float64).ravel(order="F").copy()
+    piv = np.zeros(n, np.int64)
+    ier = lib().or_getrf(n, _p(cm), piv.ctypes.data_as(C.POINTER(C.c_long)))
+    return cm, piv, ier
+
+
+def getrs(lu_cm: np.ndarray, piv: np.ndarray, b: np.ndarray) -> np.ndarray:
+    n = piv.shape[0]
+    x = np.array(b, np.float64, copy=True)
+    lib().or_getrs(n, _p(lu_cm), piv.ctypes.data_as(C.POINTER(C.c_long)), _p(x))
+    return x
+
+
+def cvode_generic(f, jac, y0, t0, tout, rtol=1e-6, atol=1e-10, maxsteps=100000,
+                  lu_fp32=False, refine_sweeps=0):
+    """BDF on a Python-defined problem: f(t, y) -> dy, jac(t, y) -> J (row-major)."""
+    n = len(y0)
+
+    def _f(t, yp, dyp, ud):
+        yv = np.ctypeslib.as_array(yp, (n,))
+        dv = np.ctypeslib.as_array(dyp, (n,))
+        dv[:] = f(t, yv.copy())
+        return 0
+
+    def _j(t, yp, fyp, Jp, ud):
+        yv = np.ctypeslib.as_array(yp, (n,))
+        Jv = np.ctypeslib.as_array(Jp, (n * n,))
+        Jv[:] = np.asarray(jac(t, yv.copy()), np.float64).T.ravel()
+        return 0
+
+    cf, cj = RHS_FN(_f), JAC_FN(_j)
+    y = np.array(y0, np.float64, copy=True)
+    o = OrOpts(rtol, atol, maxsteps, int(lu_fp32), refine_sweeps)
+    st = OrStats()
+    rc = lib().or_cvode_generic(n, cf, cj, None, t0, tout, _p(y), C.byref(o), C.byref(st))
+    return y, rc, {f_: getattr(st, f_) for f_ in STATS_FIELDS}
+
+
+# --- the reference's own pyJac C, compiled from its sources -------------------
+_pyjac = None
+
+
+def pyjac_available() -> bool:
+    return os.path.exists(PYJAC_PATH)
+
+
+def pyjac():
+    """Bindings of src/Mechanism/CPU dydt(t,P,y,dy) and eval_jacob(t,P,y,jac) (GRI-3.0)."""
+    global _pyjac
+    if _pyjac is None:
+        L = C.CDLL(PYJAC_PATH)
+        L.dydt.argtypes = [C.c_double, C.c_double, _dp, _dp]
+        L.eval_jacob.argtypes = [C.c_double, C.c_double, _dp, _dp]
+        _pyjac = L
+    return _pyjac
+
+
+def pyjac_rhs(P: float, y: np.ndarray) -> np.ndarray:
+    y = np.ascontiguousarray(y, np.float64)
+    dy = np.zeros(53)
+    pyjac().dydt(0.0, P, _p(y), _p(dy))
+    return dy
+
+
+def pyjac_jac(P: float, y: np.ndarray) -> np.ndarray:
+    """Row-major J from pyJac's column-major output (jacob.c, jac[row + col*53])."""
+    y = np.ascontiguousarray(y, np.float64)
+    J = np.zeros(53 * 53)
+    pyjac().eval_jacob(0.0, P, _p(y), _p(J))
+    return J.reshape(53, 53).T.copy()

@@ -1,0 +1,102 @@
+/*
+ * CPU oracle for the stiff-chemistry BDF hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this library is the checker. Only tests/, the
+ * smoke check and bench.py's cpu_baseline leg may load it; the product path
+ * (hpc_ode_amd) never links or calls it.
+ *
+ * It restates, in plain C:
+ *   - pyJac's constant-pressure chemistry RHS (src/Mechanism/CPU/dydt.c:7-133 and the
+ *     functions it calls) in table-driven form,
+ *   - the exact analytic Jacobian of that RHS (what src/Mechanism/CPU/jacob.c:3-371
+ *     computes), column-major like SUNDIALS' dense matrices,
+ *   - SUNDIALS' dense partial-pivot LU (SUNLinSol_Dense, used at
+ *     src/Integrators/CVodeIntegrator.cpp:125-131),
+ *   - CVODE's variable-order BDF with Newton iteration, as configured at
+ *     src/Integrators/CVodeIntegrator.cpp:98-155 (BDF, scalar tolerances, dense LS,
+ *     analytic Jacobian, maxsteps, one CV_NORMAL call from t0 = 0 to dt),
+ *   - the OpenMP cell loop of src/Integrators/CVodeIntegratorOMP.cpp:16-27.
+ */
+#ifndef HPC_ODE_ORACLE_H
+#define HPC_ODE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_SLOTS 4
+
+/* Flat mechanism tables (filled by oracle/mech.py). */
+typedef struct {
+  int K;                 /* species; state length n = K (T + K-1 mass fractions) */
+  int R;                 /* reactions */
+  const double *W;       /* [K] molecular weight kg/kmol */
+  const double *tmid;    /* [K] NASA7 mid temperature */
+  const double *nasa;    /* [K][14] low-range a0..a6, then high-range a0..a6 */
+  const int *kind;       /* [R] 0 elementary, 1 three-body, 2 Lindemann falloff, 3 Troe falloff */
+  const int *rev;        /* [R] reversible flag */
+  const int *nreac, *nprod;                 /* [R] */
+  const int *reac_sp, *prod_sp;             /* [R][OR_MAX_SLOTS] */
+  const double *reac_nu, *prod_nu;          /* [R][OR_MAX_SLOTS] */
+  const double *arr;     /* [R][8]: lnA, b, theta, A, lnA0, b0, theta0, A0 (SI-kmol, K) */
+  const double *troe;    /* [R][5]: a, T3, T1, T2, has_T2 */
+  const int *eff_ptr;    /* [R+1] */
+  const int *eff_sp;     /* [E] */
+  const double *eff_val; /* [E] efficiency minus default efficiency */
+  const double *eff_base;/* [R] default efficiency */
+} or_mech;
+
+typedef struct {
+  double rtol, atol;
+  int maxsteps;
+  int lu_fp32;           /* 0: FP64 LU; 1: FP32 LU + FP64 residual refinement */
+  int refine_sweeps;     /* refinement sweeps per linear solve when lu_fp32 */
+} or_opts;
+
+typedef struct {
+  long nst, nfe, nje, nsetups, nni, ncfn, netf, nlu, nsolve;
+  int qlast;
+  double hlast;
+} or_stats;
+
+/* CVODE-like return codes */
+#define OR_SUCCESS 0
+#define OR_TOO_MUCH_WORK -1
+#define OR_TOO_MUCH_ACC -2
+#define OR_ERR_FAILURE -3
+#define OR_CONV_FAILURE -4
+#define OR_ILL_INPUT -22
+#define OR_TOO_CLOSE -27
+
+/* Chemistry: y = [T, Y_0 .. Y_{K-2}] */
+void or_rhs(const or_mech *m, double P, const double *y, double *dy);
+/* J column-major: J[row + col*n] = d dy[row] / d y[col] */
+void or_jac(const or_mech *m, double P, const double *y, double *J);
+/* Per-species/per-reaction intermediates for unit tests:
+ * conc[K], fwd[R], rev[R], presmod[R], wdot[K] */
+void or_rates(const or_mech *m, double P, const double *y, double *conc, double *fwd,
+              double *rev, double *presmod, double *wdot);
+
+/* Dense LU, column-major, SUNDIALS denseGETRF/denseGETRS semantics. */
+long or_getrf(long n, double *a, long *p);
+void or_getrs(long n, const double *a, const long *p, double *b);
+
+/* Generic BDF driver for tests with closed-form problems. */
+typedef int (*or_rhs_fn)(double t, const double *y, double *dy, void *ud);
+typedef int (*or_jac_fn)(double t, const double *y, const double *fy, double *J, void *ud);
+int or_cvode_generic(int n, or_rhs_fn f, or_jac_fn jac, void *ud, double t0, double tout,
+                     double *y, const or_opts *o, or_stats *st);
+
+/* One chemistry cell: y in/out (length K), integrates 0 -> dt at pressure P. */
+int or_cvode_cell(const or_mech *m, double P, double dt, double *y, const or_opts *o,
+                  or_stats *st);
+
+/* Cells in parallel (OpenMP, schedule(runtime)); y is [ncells][K]; P/dt per cell.
+ * nthreads <= 0 keeps the OpenMP default. status/stats may be NULL. */
+void or_cvode_batch(const or_mech *m, long ncells, const double *P, const double *dt,
+                    double *y, const or_opts *o, int *status, or_stats *stats, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
