@@ -1,0 +1,235 @@
+"""hpc_ode_amd: MI355X-native batched stiff-chemistry BDF integration.
+
+The product is the C ABI in include/hpc_ode.h, implemented by libhpc_ode.so
+(hand-written HIP kernels for gfx950 plus the C++ host code). This module is a
+thin ctypes binding of that ABI for Python callers (tests, benchmarks, the
+serving loop). It never falls back to a CPU path: if the library is missing it
+raises, and every call goes to the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhpc_ode.so")
+NSTATS = 8
+STATS_FIELDS = ("nst", "nfe", "nje", "nsetups", "nni", "ncfn", "netf", "qlast")
+
+CELL_STATUS = {0: "SUCCESS", -1: "TOO_MUCH_WORK", -2: "TOO_MUCH_ACC", -3: "ERR_FAILURE",
+               -4: "CONV_FAILURE", -22: "ILL_INPUT", -27: "TOO_CLOSE"}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class HpoError(RuntimeError):
+    pass
+
+
+class Opts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("maxsteps", C.c_int32),
+                ("lu_precision", C.c_int32), ("refine_sweeps", C.c_int32),
+                ("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("sort_cells", C.c_int32), ("reserved", C.c_int32 * 2)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("ncells", C.c_int64),
+                ("grid", C.c_int32), ("block", C.c_int32), ("lds_bytes", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libhpc_ode.so (building it first if it does not exist)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build as _b
+            _b.build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.hpo_last_error.restype = C.c_char_p
+        L.hpo_opts_default.argtypes = [C.POINTER(Opts)]
+        L.hpo_mech_load_yaml.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.hpo_mech_destroy.argtypes = [vp]
+        L.hpo_mech_info.argtypes = [vp, _ip, _ip, _ip]
+        L.hpo_mech_species_name.argtypes = [vp, C.c_int32, C.c_char_p, C.c_int32]
+        L.hpo_mech_molecular_weights.argtypes = [vp, _dp]
+        L.hpo_mech_tables.argtypes = [vp, _dp, _dp, _dp, _ip, _ip]
+        L.hpo_batch_create.argtypes = [vp, C.c_int64, C.POINTER(vp)]
+        L.hpo_batch_destroy.argtypes = [vp]
+        L.hpo_batch_put.argtypes = [vp, C.c_int64, C.c_int64, _dp, C.c_int64, _dp, C.c_double, _dp,
+                                    C.c_double]
+        L.hpo_batch_integrate.argtypes = [vp, C.c_int64, C.POINTER(Opts), vp, C.POINTER(Timing)]
+        L.hpo_batch_get.argtypes = [vp, C.c_int64, C.c_int64, _dp, C.c_int64, _ip, _ip]
+        L.hpo_integrate_device.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp,
+                                           C.POINTER(Opts), vp, C.POINTER(Timing)]
+        L.hpo_eval_rhs.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
+        L.hpo_eval_jac.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
+        L.hpo_lu_solve.argtypes = [C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip]
+        L.hpo_device_info.argtypes = [_ip, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HpoError(f"hpc_ode error {rc}: {lib().hpo_last_error().decode()}")
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def opts(rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0, refine_sweeps=1,
+         waves_per_block=0, blocks_per_cu=0, sort_cells=0) -> Opts:
+    o = Opts()
+    lib().hpo_opts_default(C.byref(o))
+    o.rtol, o.atol, o.maxsteps = rtol, atol, maxsteps
+    o.lu_precision, o.refine_sweeps = lu_precision, refine_sweeps
+    o.waves_per_block, o.blocks_per_cu, o.sort_cells = waves_per_block, blocks_per_cu, sort_cells
+    return o
+
+
+class Mechanism:
+    """A chemistry mechanism loaded from a Cantera YAML file (the reference's
+    resources/mechanisms/*.yaml), with its tables resident on the GPU."""
+
+    def __init__(self, path: str):
+        h = C.c_void_p()
+        _check(lib().hpo_mech_load_yaml(path.encode(), C.byref(h)))
+        self._h = h
+        self.path = path
+        K, n, R = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().hpo_mech_info(h, C.byref(K), C.byref(n), C.byref(R)))
+        self.n_species, self.n_state, self.n_reactions = K.value, n.value, R.value
+        buf = C.create_string_buffer(64)
+        self.species = []
+        for k in range(self.n_species):
+            _check(lib().hpo_mech_species_name(h, k, buf, 64))
+            self.species.append(buf.value.decode())
+        self.W = np.zeros(self.n_species)
+        _check(lib().hpo_mech_molecular_weights(h, _p(self.W)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.hpo_mech_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def tables(self):
+        K, R = self.n_species, self.n_reactions
+        nasa = np.zeros((K, 14))
+        tmid = np.zeros(K)
+        arr = np.zeros((R, 8))
+        kind = np.zeros(R, np.int32)
+        rev = np.zeros(R, np.int32)
+        _check(lib().hpo_mech_tables(self._h, _p(nasa), _p(tmid), _p(arr), _p(kind, _ip), _p(rev, _ip)))
+        return dict(nasa=nasa, tmid=tmid, arr=arr, kind=kind, rev=rev)
+
+    # --- component evaluations (GPU) -------------------------------------
+    def rhs(self, y, P):
+        y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
+        n = y.shape[0]
+        P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
+        f = np.zeros_like(y)
+        _check(lib().hpo_eval_rhs(self._h, n, _p(y), _p(P), _p(f)))
+        return f
+
+    def jac(self, y, P):
+        y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
+        n = y.shape[0]
+        P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
+        J = np.zeros((n, self.n_state, self.n_state))
+        _check(lib().hpo_eval_jac(self._h, n, _p(y), _p(P), _p(J)))
+        return J
+
+    # --- integration ------------------------------------------------------
+    def integrate(self, y, P, dt, rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0,
+                  refine_sweeps=1, waves_per_block=0, blocks_per_cu=0, sort_cells=0,
+                  return_timing=False):
+        """Integrates every row of y ([ncells, n_state]) from t=0 to dt on the GPU.
+        Returns (y_out, status, stats[, timing])."""
+        y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
+        n = y.shape[0]
+        b = Batch(self, n)
+        b.put(y, P, dt)
+        t = b.integrate(n, opts(rtol, atol, maxsteps, lu_precision, refine_sweeps, waves_per_block,
+                                blocks_per_cu, sort_cells))
+        yo, st, stats = b.get(n)
+        return (yo, st, stats, t) if return_timing else (yo, st, stats)
+
+
+class Batch:
+    """Device-resident cell states (hpo_batch)."""
+
+    def __init__(self, mech: Mechanism, capacity: int):
+        self.mech = mech
+        h = C.c_void_p()
+        _check(lib().hpo_batch_create(mech.handle, capacity, C.byref(h)))
+        self._h = h
+        self.capacity = capacity
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.hpo_batch_destroy(h)
+            self._h = None
+
+    def put(self, y, P, dt, first=0):
+        y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
+        n = y.shape[0]
+        Pa = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
+        da = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (n,)))
+        _check(lib().hpo_batch_put(self._h, first, n, _p(y), y.shape[1], _p(Pa), 0.0, _p(da), 0.0))
+
+    def integrate(self, count, o: Opts | None = None, stream=None) -> Timing:
+        t = Timing()
+        _check(lib().hpo_batch_integrate(self._h, count, C.byref(o) if o is not None else None,
+                                         stream, C.byref(t)))
+        return t
+
+    def get(self, count, first=0):
+        n = self.mech.n_state
+        y = np.zeros((count, n))
+        st = np.zeros(count, np.int32)
+        stats = np.zeros((count, NSTATS), np.int32)
+        _check(lib().hpo_batch_get(self._h, first, count, _p(y), n, _p(st, _ip), _p(stats, _ip)))
+        return y, st, {f: stats[:, i].copy() for i, f in enumerate(STATS_FIELDS)}
+
+
+def lu_solve(M, b, fp32=False):
+    """Batched LU + solve with the kernel's register LU (parity tests)."""
+    M = np.ascontiguousarray(M, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    n, ns, _ = M.shape
+    x = np.zeros((n, ns))
+    ok = np.zeros(n, np.int32)
+    _check(lib().hpo_lu_solve(ns, int(fp32), n, _p(M), _p(b), _p(x), _p(ok, _ip)))
+    return x, ok
+
+
+def device_info():
+    cu = C.c_int32()
+    fr, to = C.c_int64(), C.c_int64()
+    _check(lib().hpo_device_info(C.byref(cu), C.byref(fr), C.byref(to)))
+    return dict(n_cu=cu.value, free_bytes=fr.value, total_bytes=to.value)
+
+
+def exported_symbols():
+    """Names every ABI function declared in include/hpc_ode.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "hpc_ode.h")
+    with open(hdr) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"\b(hpo_[a-z_0-9]+)\s*\(", txt)))

@@ -1,0 +1,631 @@
+// Device building blocks for the per-cell BDF kernel on CDNA4 (gfx950).
+//
+// Layout: one wave64 integrates one cell. Lane l owns state component l
+// (y = [T, Y_0 .. Y_{K-2}], as in src/Integrators/CVodeIntegrator.cpp:64-77), the
+// matching row of the Newton matrix, and the matching entries of every BDF vector.
+// The mechanism tables are staged once per workgroup into LDS; per-wave LDS holds the
+// concentrations, Gibbs energies and reaction rates of the current RHS evaluation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mechanism.hpp"
+
+namespace hpo {
+namespace dev {
+
+#define HPO_DI __device__ __forceinline__
+
+constexpr double kR = 8314.4621;
+constexpr double kPatm = 101325.0;
+
+// ---------------------------------------------------------------------------
+// cross-lane helpers (all lanes of the wave must be active)
+// ---------------------------------------------------------------------------
+HPO_DI double rl(double v, int lane) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+HPO_DI float rl(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
+HPO_DI int rli(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+template <int CTRL>
+HPO_DI double dpp(double v) {
+  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+HPO_DI float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over all 64 lanes, identical result in every lane. quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror leave each row of 16 holding its
+// sum; the four row sums are combined through readlane.
+HPO_DI double wsum(double v) {
+  v += dpp<0xB1>(v);
+  v += dpp<0x4E>(v);
+  v += dpp<0x141>(v);
+  v += dpp<0x140>(v);
+  return (rl(v, 0) + rl(v, 16)) + (rl(v, 32) + rl(v, 48));
+}
+template <typename F>
+HPO_DI F wmax(F v) {
+  v = fmax(v, dpp<0xB1>(v));
+  v = fmax(v, dpp<0x4E>(v));
+  v = fmax(v, dpp<0x141>(v));
+  v = fmax(v, dpp<0x140>(v));
+  return fmax(fmax(rl(v, 0), rl(v, 16)), fmax(rl(v, 32), rl(v, 48)));
+}
+
+HPO_DI double shfl(double v, int src) { return __shfl(v, src, 64); }
+
+// Orders LDS and scratch accesses of the lanes of one wave (no workgroup barrier).
+HPO_DI void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int N, typename T>
+HPO_DI T pick(const T (&v)[N], int i) {
+  T r = v[0];
+#pragma unroll
+  for (int j = 1; j < N; ++j) r = (i == j) ? v[j] : r;
+  return r;
+}
+template <int N, typename T>
+HPO_DI void put(T (&v)[N], int i, T x) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = (i == j) ? x : v[j];
+}
+
+// ---------------------------------------------------------------------------
+// table views
+// ---------------------------------------------------------------------------
+struct Tabs {
+  const double* sp;        // [K][kSpStride]
+  const RxnHdr* hdr;       // [R]
+  const double* arr;       // [R][3]
+  const PdepRec* pd;       // [n_pdep]
+  const double* effv;      // [n_eff]
+  const uint8_t* effs;     // [n_eff]
+  const double* coef;      // [16]
+  const uint32_t* gather;  // [ngl][64]
+  const uint16_t* pptr;    // [K+1]
+  const uint16_t* slot_base;  // [R+1] (global)
+  const uint64_t* jterm;      // [njl][64] (global)
+  int K, R, ngl, njl, npart;
+};
+
+HPO_DI Tabs make_tabs(const uint8_t* lds, const uint8_t* glob, const DevMechLayout& L) {
+  Tabs t;
+  t.sp = reinterpret_cast<const double*>(lds + L.off_species);
+  t.hdr = reinterpret_cast<const RxnHdr*>(lds + L.off_hdr);
+  t.arr = reinterpret_cast<const double*>(lds + L.off_arr);
+  t.pd = reinterpret_cast<const PdepRec*>(lds + L.off_pdep);
+  t.effv = reinterpret_cast<const double*>(lds + L.off_eff_val);
+  t.effs = lds + L.off_eff_sp;
+  t.coef = reinterpret_cast<const double*>(lds + L.off_coef);
+  t.gather = reinterpret_cast<const uint32_t*>(lds + L.off_gather);
+  t.pptr = reinterpret_cast<const uint16_t*>(lds + L.off_part_ptr);
+  t.slot_base = reinterpret_cast<const uint16_t*>(glob + L.off_slot_base);
+  t.jterm = reinterpret_cast<const uint64_t*>(glob + L.off_jterm);
+  t.K = L.K;
+  t.R = L.R;
+  t.ngl = L.n_gather_lane;
+  t.njl = L.n_jterm_lane;
+  t.npart = L.n_partials;
+  return t;
+}
+
+// per-wave LDS: C[64], g[64], hRT[64], cp[64] (mass), phi[R], part[3*npart]
+struct WaveLds {
+  double *C, *g, *hRT, *cp, *phi, *part;
+};
+__host__ __device__ inline int wave_lds_doubles(int R, int npart) { return 256 + ((R + 1) & ~1) + 3 * ((npart + 1) & ~1); }
+HPO_DI WaveLds make_wave_lds(double* base, int R, int npart) {
+  WaveLds w;
+  w.C = base;
+  w.g = base + 64;
+  w.hRT = base + 128;
+  w.cp = base + 192;
+  w.phi = base + 256;
+  w.part = base + 256 + ((R + 1) & ~1);
+  return w;
+}
+
+// per-wave global scratch (doubles)
+struct WaveScratch {
+  double *dphi, *coefbuf, *gphi, *gdm, *gdT, *JC, *J;
+};
+HPO_DI WaveScratch make_wave_scratch(double* base, const DevMechLayout& L) {
+  WaveScratch s;
+  int o = 0;
+  s.JC = base + o;
+  o += 64 * 64;
+  s.J = base + o;
+  o += 64 * 64;
+  s.dphi = base + o;
+  o += (L.n_slots + 1) & ~1;
+  s.coefbuf = base + o;
+  o += 16 + ((L.R + 1) & ~1);
+  s.gphi = base + o;
+  o += (L.R + 1) & ~1;
+  s.gdm = base + o;
+  o += (L.R + 1) & ~1;
+  s.gdT = base + o;
+  return s;
+}
+inline int64_t wave_scratch_doubles(const DevMechLayout& L) {
+  int64_t o = 2 * 64 * 64 + ((L.n_slots + 1) & ~1) + 16 + 4 * ((L.R + 1) & ~1);
+  return (o + 31) & ~int64_t(31);
+}
+
+// ---------------------------------------------------------------------------
+// species-level state shared by the RHS and the Jacobian
+// ---------------------------------------------------------------------------
+struct SpState {
+  double T, lnT, invT, P, rho, Wbar, m, lnPfac;
+  double Yk, invW, W, cpR, hRT, sR, dcpR;  // per lane (species k = lane)
+};
+
+HPO_DI void species_state(const Tabs& t, double y, double P, int lane, SpState& s) {
+  const int K = t.K;
+  const double T = rl(y, 0);
+  s.T = T;
+  s.P = P;
+  s.invT = 1.0 / T;
+  s.lnT = log(T);
+  double Yk = shfl(y, (lane + 1) & 63);
+  double ysum = wsum((lane >= 1 && lane < K) ? y : 0.0);
+  if (lane == K - 1) Yk = 1.0 - ysum;
+  if (lane >= K) Yk = 0.0;
+  const double* rec = t.sp + (lane < K ? lane : K - 1) * kSpStride;
+  s.invW = rec[0];
+  s.W = rec[1];
+  s.Yk = Yk;
+  double S = wsum(Yk * s.invW);
+  s.Wbar = 1.0 / S;
+  s.rho = P * s.Wbar / (kR * T);
+  s.m = P / (kR * T);
+  s.lnPfac = log(kPatm / (kR * T));
+  const double* a = rec + 3 + (T <= rec[2] ? 0 : 7);
+  s.cpR = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
+  s.hRT = a[0] + T * (a[1] * 0.5 + T * (a[2] * (1.0 / 3.0) + T * (a[3] * 0.25 + T * a[4] * 0.2))) + a[5] * s.invT;
+  s.sR = a[0] * s.lnT + T * (a[1] + T * (a[2] * 0.5 + T * (a[3] * (1.0 / 3.0) + T * a[4] * 0.25))) + a[6];
+  s.dcpR = a[1] + T * (2.0 * a[2] + T * (3.0 * a[3] + T * 4.0 * a[4]));
+}
+
+HPO_DI double cpow_int(double c, int nu) {
+  double p = c;
+  if (nu >= 2) p = c * c;
+  if (nu >= 3) p *= c;
+  return p;
+}
+HPO_DI double dcpow_int(double c, int nu) { return nu == 1 ? 1.0 : (nu == 2 ? 2.0 * c : 3.0 * c * c); }
+
+HPO_DI double arrh(double lnA, double b, double th, bool constk, double lnT, double invT) {
+  return constk ? lnA : exp(lnA + b * lnT - th * invT);
+}
+
+// Balanced gather: out[k] = sum_r nu_kr src[r] for species k = lane (0 for lane >= K).
+// Each lane walks its share of the nonzeros, flushing partial sums per species run;
+// lane k then adds the partials of species k.
+template <int NSRC>
+HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part, int lane, double (&out)[NSRC]) {
+  double acc[NSRC];
+#pragma unroll
+  for (int s = 0; s < NSRC; ++s) acc[s] = 0.0;
+#pragma unroll 1
+  for (int i = 0; i < t.ngl; ++i) {
+    uint32_t e = t.gather[i * 64 + lane];
+    double c = t.coef[(e >> 12) & 15u];
+    int r = static_cast<int>(e & 4095u);
+#pragma unroll
+    for (int s = 0; s < NSRC; ++s) acc[s] += c * src[s][r];
+    if (e & (1u << 16)) {
+      int p = static_cast<int>(e >> 17);
+#pragma unroll
+      for (int s = 0; s < NSRC; ++s) {
+        part[s * t.npart + p] = acc[s];
+        acc[s] = 0.0;
+      }
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int s = 0; s < NSRC; ++s) out[s] = 0.0;
+  if (lane < t.K) {
+    int b = t.pptr[lane], e = t.pptr[lane + 1];
+    for (int p = b; p < e; ++p) {
+#pragma unroll
+      for (int s = 0; s < NSRC; ++s) out[s] += part[s * t.npart + p];
+    }
+  }
+  wave_sync();
+}
+
+// ---------------------------------------------------------------------------
+// RHS: constant-pressure dydt (src/Mechanism/CPU/dydt.c:7-133 in table form)
+// ---------------------------------------------------------------------------
+HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane, int n) {
+  const int K = t.K, R = t.R;
+  SpState s;
+  species_state(t, y, P, lane, s);
+  if (lane < K) {
+    w.C[lane] = s.rho * s.Yk * s.invW;
+    w.g[lane] = s.hRT - s.sR;
+  }
+  wave_sync();
+#pragma unroll 1
+  for (int r = lane; r < R; r += 64) {
+    const RxnHdr h = t.hdr[r];
+    const double* ar = t.arr + 3 * r;
+    double kf = arrh(ar[0], ar[1], ar[2], h.flags & 2, s.lnT, s.invT);
+    double fwd = kf;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (h.sp[q] != 0xFF) fwd *= cpow_int(w.C[h.sp[q]], h.nu[q]);
+    double rev = 0.0;
+    if (h.flags & 1) {
+      double dG = 0.0, dnu = 0.0, pc = 1.0;
+#pragma unroll
+      for (int q = 3; q < 6; ++q)
+        if (h.sp[q] != 0xFF) {
+          dG += h.nu[q] * w.g[h.sp[q]];
+          dnu += h.nu[q];
+          pc *= cpow_int(w.C[h.sp[q]], h.nu[q]);
+        }
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (h.sp[q] != 0xFF) {
+          dG -= h.nu[q] * w.g[h.sp[q]];
+          dnu -= h.nu[q];
+        }
+      // k_r = k_f / Kc, Kc = (P_atm/(R T))^dnu exp(-dG)
+      rev = kf * exp(dG - dnu * s.lnPfac) * pc;
+    }
+    double pm = 1.0;
+    if (h.kind != kElementary) {
+      const PdepRec& pr = t.pd[h.pidx];
+      double M = pr.eff_base * s.m;
+      for (int e = pr.eff_begin; e < pr.eff_end; ++e) M += t.effv[e] * w.C[t.effs[e]];
+      if (h.kind == kThreeBody) {
+        pm = M;
+      } else {
+        double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, h.flags & 8, s.lnT, s.invT);
+        double Pr = k0 * M / kf;
+        double F = 1.0;
+        if (h.kind == kTroe) {
+          double Fc = (1.0 - pr.troe_a) * exp(-s.T / pr.troe_T3) + pr.troe_a * exp(-s.T / pr.troe_T1) +
+                      ((h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0);
+          double logFc = log10(fmax(Fc, 1.0e-300));
+          double x = log10(fmax(Pr, 1.0e-300));
+          double A = x - 0.67 * logFc - 0.4;
+          double B = 0.806 - 1.1762 * logFc - 0.14 * x;
+          F = exp10(logFc / (1.0 + A * A / (B * B)));
+        }
+        pm = F * Pr / (1.0 + Pr);
+      }
+    }
+    w.phi[r] = (fwd - rev) * pm;
+  }
+  wave_sync();
+  const double* src[1] = {w.phi};
+  double wd[1];
+  gather<1>(t, src, w.part, lane, wd);
+  const double wdot = wd[0];
+  const double Hk = s.hRT * kR * s.T;  // molar enthalpy
+  double Q = wsum(lane < K ? Hk * wdot : 0.0);
+  double cpmix = wsum(lane < K ? s.Yk * s.cpR * kR * s.invW : 0.0);
+  double fY = wdot * s.W / s.rho;
+  double f = shfl(fY, (lane + 63) & 63);
+  if (lane == 0) f = -Q / (s.rho * cpmix);
+  if (lane >= n) f = 0.0;
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+// Analytic Jacobian (what src/Mechanism/CPU/jacob.c:3-371 computes), one row per lane.
+// ---------------------------------------------------------------------------
+// The row of lane l is written to g.J[j*64 + l], j = 0..NS-1.
+template <int NS>
+HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, double y, double P, int lane) {
+  const int K = t.K, R = t.R;
+  SpState s;
+  species_state(t, y, P, lane, s);
+  const double cpk = s.cpR * kR * s.invW;  // mass cp of species k
+  if (lane < K) {
+    w.C[lane] = s.rho * s.Yk * s.invW;
+    w.g[lane] = s.hRT - s.sR;
+    w.hRT[lane] = s.hRT;
+    w.cp[lane] = cpk;
+  }
+  // zero the dense d wdot/dC buffer (columns 0..K-1, rows 0..63)
+  for (int i = 0; i < K; ++i) g.JC[i * 64 + lane] = 0.0;
+  wave_sync();
+  const double RT = kR * s.T;
+#pragma unroll 1
+  for (int r = lane; r < R; r += 64) {
+    const RxnHdr h = t.hdr[r];
+    const double* ar = t.arr + 3 * r;
+    const bool constk = h.flags & 2;
+    double kf = arrh(ar[0], ar[1], ar[2], constk, s.lnT, s.invT);
+    double dlnkf = constk ? 0.0 : (ar[1] + ar[2] * s.invT) * s.invT;
+    double cr[3], cp_[3];
+    double fwd = kf;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      cr[q] = (h.sp[q] != 0xFF) ? w.C[h.sp[q]] : 1.0;
+      if (h.sp[q] != 0xFF) fwd *= cpow_int(cr[q], h.nu[q]);
+    }
+    double kr = 0.0, rev = 0.0, dlnkr = 0.0, dH = 0.0;
+    {
+      double dG = 0.0, dnu = 0.0, dHRT = 0.0;
+#pragma unroll
+      for (int q = 3; q < 6; ++q) {
+        cp_[q - 3] = (h.sp[q] != 0xFF) ? w.C[h.sp[q]] : 1.0;
+        if (h.sp[q] != 0xFF) {
+          dG += h.nu[q] * w.g[h.sp[q]];
+          dHRT += h.nu[q] * w.hRT[h.sp[q]];
+          dnu += h.nu[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (h.sp[q] != 0xFF) {
+          dG -= h.nu[q] * w.g[h.sp[q]];
+          dHRT -= h.nu[q] * w.hRT[h.sp[q]];
+          dnu -= h.nu[q];
+        }
+      dH = dHRT * RT;
+      if (h.flags & 1) {
+        kr = kf * exp(dG - dnu * s.lnPfac);
+        rev = kr;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          if (h.sp[3 + q] != 0xFF) rev *= cpow_int(cp_[q], h.nu[3 + q]);
+        dlnkr = dlnkf - (dHRT - dnu) * s.invT;
+      }
+    }
+    double pm = 1.0, dpm_dM = 0.0, dpm_dT = 0.0, eff_base = 0.0;
+    int eb = 0, ee = 0;
+    if (h.kind != kElementary) {
+      const PdepRec& pr = t.pd[h.pidx];
+      eff_base = pr.eff_base;
+      eb = pr.eff_begin;
+      ee = pr.eff_end;
+      double M = pr.eff_base * s.m;
+      for (int e = eb; e < ee; ++e) M += t.effv[e] * w.C[t.effs[e]];
+      if (h.kind == kThreeBody) {
+        pm = M;
+        dpm_dM = 1.0;
+      } else {
+        const bool constk0 = h.flags & 8;
+        double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, constk0, s.lnT, s.invT);
+        double dlnk0 = constk0 ? 0.0 : (pr.b0 + pr.theta0 * s.invT) * s.invT;
+        double Pr = k0 * M / kf;
+        double dPr_dT = Pr * (dlnk0 - dlnkf);
+        double dPr_dM = k0 / kf;
+        double qq = 1.0 / (1.0 + Pr);
+        if (h.kind == kTroe) {
+          double e3 = exp(-s.T / pr.troe_T3), e1 = exp(-s.T / pr.troe_T1);
+          double e2 = (h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0;
+          double Fc = (1.0 - pr.troe_a) * e3 + pr.troe_a * e1 + e2;
+          double dFc = -(1.0 - pr.troe_a) / pr.troe_T3 * e3 - pr.troe_a / pr.troe_T1 * e1 +
+                       ((h.flags & 4) ? pr.troe_T2 * s.invT * s.invT * e2 : 0.0);
+          bool fcl = !(Fc > 1.0e-300), prl = !(Pr > 1.0e-300);
+          double logFc = log10(fmax(Fc, 1.0e-300));
+          double x = log10(fmax(Pr, 1.0e-300));
+          double A = x - 0.67 * logFc - 0.4;
+          double B = 0.806 - 1.1762 * logFc - 0.14 * x;
+          double D = 1.0 + A * A / (B * B);
+          double F = exp10(logFc / D);
+          double dD_dA = 2.0 * A / (B * B), dD_dB = -2.0 * A * A / (B * B * B);
+          double df_dx = prl ? 0.0 : -logFc / (D * D) * (dD_dA - 0.14 * dD_dB);
+          double df_dc = 1.0 / D - logFc / (D * D) * (-0.67 * dD_dA - 1.1762 * dD_dB);
+          double dc_dT = fcl ? 0.0 : dFc / (Fc * 2.302585092994046);
+          double dF_dT = F * 2.302585092994046 * df_dc * dc_dT;
+          pm = F * Pr * qq;
+          double dpm_dPr = F * qq * qq + F * df_dx * qq;
+          dpm_dM = dpm_dPr * dPr_dM;
+          dpm_dT = dpm_dPr * dPr_dT + Pr * qq * dF_dT;
+        } else {
+          pm = Pr * qq;
+          dpm_dM = qq * qq * dPr_dM;
+          dpm_dT = qq * qq * dPr_dT;
+        }
+      }
+    }
+    const double dq = fwd - rev;
+    int sb = t.slot_base[r];
+    // reactant slots: pm * d fwd / dC
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (h.sp[a] != 0xFF) {
+        double d = kf;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          if (h.sp[b] != 0xFF) d *= (b == a) ? dcpow_int(cr[b], h.nu[b]) : cpow_int(cr[b], h.nu[b]);
+        g.dphi[sb++] = pm * d;
+      }
+    if (h.flags & 1) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (h.sp[3 + a] != 0xFF) {
+          double d = kr;
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
+            if (h.sp[3 + b] != 0xFF) d *= (b == a) ? dcpow_int(cp_[b], h.nu[3 + b]) : cpow_int(cp_[b], h.nu[3 + b]);
+          g.dphi[sb++] = -pm * d;
+        }
+    }
+    const double fM = dq * dpm_dM;
+    for (int e = eb; e < ee; ++e) g.dphi[sb++] = fM * t.effv[e];
+    g.gphi[r] = dq * pm;
+    g.gdm[r] = fM * eff_base;
+    g.gdT[r] = pm * (fwd * dlnkf - rev * dlnkr) + dq * dpm_dT;
+    g.coefbuf[16 + r] = dH;
+  }
+  wave_sync();
+  // owner-computes assembly of d wdot / dC (rows 0..K-2) and sum_k H_k d wdot_k / dC (row K)
+  {
+    double acc = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < t.njl; ++i) {
+      uint64_t e = t.jterm[i * 64 + lane];
+      int cell = static_cast<int>(e & 0xFFFFu);
+      int src = static_cast<int>((e >> 16) & 0xFFFFu);
+      int ci = static_cast<int>((e >> 32) & 0xFFFFu);
+      acc += g.coefbuf[ci] * g.dphi[src];
+      if ((e >> 48) & 1u) {
+        g.JC[cell] = acc;
+        acc = 0.0;
+      }
+    }
+  }
+  // species gathers of wdot, d wdot/dT|C,m and d wdot/dm
+  const double* srcs[3] = {g.gphi, g.gdT, g.gdm};
+  double gv[3];
+  gather<3>(t, srcs, w.part, lane, gv);  // contains wave_sync
+  const double wdot = gv[0], JT = gv[1], Jm = gv[2];
+  const double Hk = s.hRT * RT;
+  const bool sp = lane < K;
+  const double Q = wsum(sp ? Hk * wdot : 0.0);
+  const double SCpw = wsum(sp ? s.cpR * kR * wdot : 0.0);
+  const double SHJT = wsum(sp ? Hk * JT : 0.0);
+  const double SHJm = wsum(sp ? Hk * Jm : 0.0);
+  const double cpmix = wsum(sp ? s.Yk * cpk : 0.0);
+  const double dcpmix = wsum(sp ? s.Yk * s.dcpR * kR * s.invW : 0.0);
+  // species row k = lane-1 lives in lane k+1
+  const double wdot_k = shfl(wdot, (lane + 63) & 63);
+  const double JT_k = shfl(JT, (lane + 63) & 63);
+  const double Jm_k = shfl(Jm, (lane + 63) & 63);
+  // this lane's row of the buffer: species row lane-1, or the enthalpy-weighted row K
+  const double* jrow = g.JC + ((lane == 0) ? K : ((lane <= K) ? lane - 1 : 63));
+  double sigma = 0.0;
+#pragma unroll 4
+  for (int i = 0; i < NS; ++i) sigma += jrow[i * 64] * w.C[i];
+  const double jcN = jrow[(NS - 1) * 64];
+  const double rho = s.rho, Wbar = s.Wbar, T = s.T;
+  const double invWN = t.sp[(K - 1) * kSpStride];
+  const double Wk = t.sp[((lane >= 1 && lane < K) ? lane - 1 : 0) * kSpStride + 1];
+  const double fac = Wk / rho;
+  const bool isT = lane == 0;
+  const bool live = lane < NS;
+  const double D = rho * cpmix;
+  const double invD = 1.0 / D;
+  const double fT = -Q * invD;
+  const double cpN = w.cp[K - 1];
+  // column 0 (temperature)
+  {
+    double dWdT = JT_k - sigma * s.invT - (s.m * s.invT) * Jm_k;
+    double spv = fac * (dWdT + wdot_k * s.invT);
+    double dQdT = SCpw + SHJT - sigma * s.invT - (s.m * s.invT) * SHJm;  // lane 0: sigma = sum_k H_k sigma_k
+    double dDdT = -rho * cpmix * s.invT + rho * dcpmix;
+    double tv = (-dQdT - fT * dDdT) * invD;
+    g.J[lane] = live ? (isT ? tv : spv) : 0.0;
+  }
+  const double c1 = jcN * rho * invWN;
+#pragma unroll 2
+  for (int j = 0; j < NS - 1; ++j) {
+    const double invWj = t.sp[j * kSpStride];
+    const double dj = invWj - invWN;
+    const double base = jrow[j * 64] * rho * invWj - c1 - Wbar * dj * sigma;
+    double spv = fac * (base + wdot_k * Wbar * dj);
+    double dDdY = -rho * Wbar * dj * cpmix + rho * (w.cp[j] - cpN);
+    double tv = (-base - fT * dDdY) * invD;
+    g.J[(j + 1) * 64 + lane] = live ? (isT ? tv : spv) : 0.0;
+  }
+  wave_sync();
+}
+
+// ---------------------------------------------------------------------------
+// Dense LU of the Newton matrix, one row per lane (SUNDIALS denseGETRF order of
+// operations with LAPACK-position pivot ties), and the matching solve.
+// ---------------------------------------------------------------------------
+HPO_DI double pivot_key(double v, int pos) {
+  unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(fabs(v)));
+  b = (b & ~63ull) | static_cast<unsigned long long>(63 - pos);
+  return __longlong_as_double(static_cast<long long>(b));
+}
+HPO_DI float pivot_key(float v, int pos) {
+  unsigned b = static_cast<unsigned>(__float_as_int(fabsf(v)));
+  b = (b & ~63u) | static_cast<unsigned>(63 - pos);
+  return __int_as_float(static_cast<int>(b));
+}
+HPO_DI int key_pos(double k) { return 63 - static_cast<int>(__double_as_longlong(k) & 63); }
+HPO_DI int key_pos(float k) { return 63 - (__float_as_int(k) & 63); }
+
+template <int NS, typename F>
+struct LuReg {
+  F a[NS];
+  F rdiag;    // 1/U_kk for the step this lane pivoted
+  int step;   // elimination step at which this lane was the pivot (64: never)
+  int perm;   // lane k: pivot lane of step k
+};
+
+template <int NS, typename F>
+HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
+  int pos = lane;
+  bool done = false;
+  L.step = 64;
+  L.perm = 0;
+  L.rdiag = F(0);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const bool cand = (!done) && (lane < NS);
+    F key = cand ? pivot_key(L.a[k], pos) : F(0);
+    F kmax = wmax(key);
+    int ppos = key_pos(kmax);
+    unsigned long long bal = __ballot(cand && pos == ppos);
+    if (bal == 0ull) return false;  // NaN column
+    int p = __ffsll(bal) - 1;
+    F piv = rl(L.a[k], p);
+    if (piv == F(0)) return false;
+    F mult = F(1) / piv;
+    if (pos == k) pos = ppos;
+    if (lane == p) {
+      pos = k;
+      done = true;
+      L.step = k;
+      L.rdiag = mult;
+    }
+    if (lane == k) L.perm = p;
+    F l = done ? F(0) : L.a[k] * mult;
+    if (!done) L.a[k] = l;
+#pragma unroll
+    for (int j = k + 1; j < NS; ++j) {
+      F pj = rl(L.a[j], p);
+      L.a[j] = fma(-l, pj, L.a[j]);
+    }
+  }
+  return true;
+}
+
+template <int NS, typename F>
+HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane) {
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    int p = rli(L.perm, k);
+    F yk = rl(b, p);
+    F c = (L.step > k) ? L.a[k] : F(0);
+    b = fma(-c, yk, b);
+  }
+  F x = F(0);
+#pragma unroll
+  for (int k = NS - 1; k >= 0; --k) {
+    int p = rli(L.perm, k);
+    F xk = rl(b, p) * rl(L.rdiag, p);
+    x = (lane == k) ? xk : x;
+    F c = (L.step < k) ? L.a[k] : F(0);
+    b = fma(-c, xk, b);
+  }
+  return x;
+}
+
+}  // namespace dev
+}  // namespace hpo

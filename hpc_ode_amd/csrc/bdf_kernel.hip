@@ -1,0 +1,784 @@
+// Persistent per-cell BDF kernel for gfx950.
+//
+// Each wave64 pulls cells from a device work queue and integrates one cell at a
+// time from t = 0 to its dt with CVODE's variable-order BDF (the solver the reference
+// configures at src/Integrators/CVodeIntegrator.cpp:98-155: BDF, scalar tolerances,
+// dense direct linear solver, analytic Jacobian, maxsteps, one CV_NORMAL call).
+// The whole integration is resident: Nordsieck history, error weights and the
+// Newton matrix row of lane l stay in registers; the RHS is evaluated at one code
+// site driven by a small state machine so the kernel is not replicated per caller.
+#include <hip/hip_runtime.h>
+#include <float.h>
+
+#include "bdf_device.hpp"
+#include "kernels.hpp"
+
+namespace hpo {
+namespace dev {
+
+// CVODE constants (SUNDIALS 6.x defaults)
+constexpr int QMAX = 5;
+constexpr double ADDON = 1.0e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0;
+constexpr double ETAMX1 = 10000.0, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2, ETAMIN = 0.1, ETACF = 0.25;
+constexpr double THRESH = 1.5;
+constexpr int SMALL_NST = 10, MXNCF = 10, MXNEF = 7, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10;
+constexpr int NLS_MAXCOR = 3, MSBP = 20, MSBJ = 51;
+constexpr double CRDOWN = 0.3, DGMAX_LSETUP = 0.3, RDIV = 2.0, DGMAX_JBAD = 0.2, NLSCOEF = 0.1;
+constexpr double HLB_FACTOR = 100.0, HUB_FACTOR = 0.1, H_BIAS = 0.5;
+constexpr int MAX_ITERS = 4;
+constexpr double UROUND = DBL_EPSILON;
+
+enum { FIRST_CALL = 0, PREV_CONV_FAIL = 1, PREV_ERR_FAIL = 2 };
+enum { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
+enum { PH_F0 = 0, PH_HIN = 1, PH_NEWTON = 2, PH_RESTART = 3 };
+enum { ACT_EVAL = 0, ACT_BEGIN = 1, ACT_ATTEMPT = 2, ACT_DONE = 3 };
+
+template <int NS, bool LU32>
+struct LuStore;
+template <int NS>
+struct LuStore<NS, false> {
+  LuReg<NS, double> L;
+};
+template <int NS>
+struct LuStore<NS, true> {
+  LuReg<NS, float> L;
+  float J[NS];  // Jacobian row kept for the FP64-arithmetic refinement residual
+};
+
+HPO_DI double wrms(double v, double w, int n) {
+  double t = v * w;
+  return sqrt(wsum(t * t) / n);
+}
+
+// Wave-uniform controller state. It lives in per-wave LDS so that the ~40 scalars
+// do not each occupy a VGPR pair next to the register-resident Newton matrix.
+struct Scal {
+  double tn, h, hprime, hscale, eta, etamax, hu;
+  double gamma, gammap, gamrat, rl1, crate, delp, acnrm, saved_tq5, saved_t, gamma_lu;
+  double hlb, hub, hg, hin_sign;
+  double tau[8], tq[6], l[6];
+};
+static_assert(sizeof(Scal) % 8 == 0, "Scal layout");
+constexpr int kScalDoubles = sizeof(Scal) / 8;
+
+template <int NS, bool LU32>
+__device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w, const WaveScratch& g, Scal& S,
+                               int64_t cell, int lane) {
+  const int n = NS;
+  const bool act = lane < n;
+  const double P = A.P[cell];
+  const double tout = A.dt[cell];
+  const double rtol = A.rtol, atol = A.atol;
+
+  double zn[QMAX + 1];
+  zn[0] = act ? A.y_in[cell * A.ld + lane] : 0.0;
+#pragma unroll
+  for (int j = 1; j <= QMAX; ++j) zn[j] = 0.0;
+  double ewt = 0.0, acor = 0.0, yv = zn[0];
+  LuStore<NS, LU32> lu;
+
+  if (lane == 0) {
+    S.tn = 0.0;
+    S.h = S.hprime = S.hscale = 0.0;
+    S.eta = 1.0;
+    S.etamax = ETAMX1;
+    S.hu = 0.0;
+    S.gamma = S.gammap = 0.0;
+    S.gamrat = 1.0;
+    S.rl1 = 1.0;
+    S.crate = 1.0;
+    S.delp = S.acnrm = S.saved_tq5 = S.saved_t = S.gamma_lu = 0.0;
+    S.hlb = S.hub = S.hg = 0.0;
+    S.hin_sign = 1.0;
+    for (int j = 0; j < 8; ++j) S.tau[j] = 0.0;
+    for (int j = 0; j < 6; ++j) S.tq[j] = S.l[j] = 0.0;
+  }
+  wave_sync();
+  int q = 1, qprime = 1, qwait = 2, L = 2, jcur = 0, convfail = NO_FAILURES;
+  int nst = 0, nfe = 0, nje = 0, nsetups = 0, nni = 0, ncfn = 0, netf = 0;
+  int nstlp = 0, nstlj = 0, nstloc = 0;
+  int ncf = 0, nef = 0, nflag = FIRST_CALL, m = 0, need_setup = 0, jbad = 0;
+  int status = HPO_SUCCESS;
+  int count1 = 0;
+
+  // error weights for y0 (cvInitialSetup)
+  {
+    double tol = rtol * fabs(zn[0]) + atol;
+    if (__ballot(act && !(tol > 0.0)) != 0ull) status = HPO_ILL_INPUT;
+    ewt = act ? 1.0 / tol : 0.0;
+  }
+
+  // All writes to S below happen identically in every lane (same value, same address),
+  // so no lane-0 guard is needed; reads see the wave's own writes in program order.
+  auto rescale = [&]() {
+    double f = S.eta;
+#pragma unroll
+    for (int j = 1; j <= QMAX; ++j) {
+      if (j <= q) {
+        zn[j] *= f;
+        f *= S.eta;
+      }
+    }
+    S.h = S.hscale * S.eta;
+    S.hscale = S.h;
+  };
+  auto restore = [&]() {
+    S.tn = S.saved_t;
+#pragma unroll
+    for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+      for (int j = QMAX; j >= k; --j)
+        if (j <= q) zn[j - 1] -= zn[j];
+  };
+  auto adjust_order = [&](int deltaq) {
+    if (q == 2 && deltaq != 1) return;
+    if (deltaq == 1) {
+      // cvIncreaseBDF
+      double alpha0 = -1.0, alpha1 = 1.0, prod = 1.0, xiold = 1.0, hsum = S.hscale;
+      for (int i = 0; i <= QMAX; ++i) S.l[i] = 0.0;
+      S.l[2] = 1.0;
+      for (int j = 1; j < q; ++j) {
+        hsum += S.tau[j + 1];
+        double xi = hsum / S.hscale;
+        prod *= xi;
+        alpha0 -= 1.0 / (j + 1);
+        alpha1 += 1.0 / xi;
+        for (int i = j + 2; i >= 2; --i) S.l[i] = S.l[i] * xiold + S.l[i - 1];
+        xiold = xi;
+      }
+      double A1 = (-alpha0 - alpha1) / prod;
+      double zl = A1 * zn[QMAX];  // zn[indx_acor], indx_acor = qmax
+      put(zn, L, zl);
+#pragma unroll
+      for (int j = 2; j <= QMAX; ++j)
+        if (j <= q) zn[j] += S.l[j] * zl;
+    } else if (deltaq == -1) {
+      // cvDecreaseBDF
+      for (int i = 0; i <= QMAX; ++i) S.l[i] = 0.0;
+      S.l[2] = 1.0;
+      double hsum = 0.0;
+      for (int j = 1; j <= q - 2; ++j) {
+        hsum += S.tau[j];
+        double xi = hsum / S.hscale;
+        for (int i = j + 2; i >= 2; --i) S.l[i] = S.l[i] * xi + S.l[i - 1];
+      }
+      double zq = pick(zn, q);
+#pragma unroll
+      for (int j = 2; j < QMAX; ++j)
+        if (j < q) zn[j] = -S.l[j] * zq + zn[j];
+    }
+  };
+  // predict + cvSetBDF + cvNls preamble; returns ACT_EVAL
+  auto attempt = [&]() -> int {
+    S.tn += S.h;
+#pragma unroll
+    for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+      for (int j = QMAX; j >= k; --j)
+        if (j <= q) zn[j - 1] += zn[j];
+    {
+      // cvSetBDF
+      const double h = S.h;
+      double alpha0 = -1.0, alpha0_hat = -1.0, xi_inv = 1.0, xistar_inv = 1.0, hsum = h;
+      S.l[0] = 1.0;
+      S.l[1] = 1.0;
+      for (int i = 2; i <= QMAX; ++i) S.l[i] = 0.0;
+      if (q > 1) {
+        for (int j = 2; j < q; ++j) {
+          hsum += S.tau[j - 1];
+          xi_inv = h / hsum;
+          alpha0 -= 1.0 / j;
+          for (int i = j; i >= 1; --i) S.l[i] += S.l[i - 1] * xi_inv;
+        }
+        alpha0 -= 1.0 / q;
+        xistar_inv = -S.l[1] - alpha0;
+        hsum += S.tau[q - 1];
+        xi_inv = h / hsum;
+        alpha0_hat = -S.l[1] - xi_inv;
+        for (int i = q; i >= 1; --i) S.l[i] += S.l[i - 1] * xistar_inv;
+      }
+      // cvSetTqBDF
+      double A1 = 1.0 - alpha0_hat + alpha0;
+      double A2 = 1.0 + q * A1;
+      double lq = S.l[q];
+      S.tq[2] = fabs(A1 / (alpha0 * A2));
+      S.tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
+      if (qwait == 1) {
+        if (q > 1) {
+          double C = xistar_inv / lq;
+          double A3 = alpha0 + 1.0 / q;
+          double A4 = alpha0_hat + xi_inv;
+          double Cpinv = (1.0 - A4 + A3) / A3;
+          S.tq[1] = fabs(C * Cpinv);
+        } else {
+          S.tq[1] = 1.0;
+        }
+        hsum += S.tau[q];
+        xi_inv = h / hsum;
+        double A5 = alpha0 - (1.0 / (q + 1));
+        double A6 = alpha0_hat - xi_inv;
+        double Cppinv = (1.0 - A6 + A5) / A2;
+        S.tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+      }
+      S.tq[4] = NLSCOEF / S.tq[2];
+      S.rl1 = 1.0 / S.l[1];
+      S.gamma = h * S.rl1;
+      if (nst == 0) S.gammap = S.gamma;
+      S.gamrat = (nst > 0) ? S.gamma / S.gammap : 1.0;
+    }
+    // cvNls preamble
+    convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
+    need_setup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (nst == 0) || (nst >= nstlp + MSBP) ||
+                 (fabs(S.gamrat - 1.0) > DGMAX_LSETUP);
+    jbad = 0;
+    m = 0;
+    acor = 0.0;
+    yv = zn[0];
+    return ACT_EVAL;
+  };
+  // top of an internal step (CVode loop body + cvStep entry); returns ACT_ATTEMPT or ACT_DONE
+  auto begin = [&]() -> int {
+    if (nst > 0) {
+      double tol = rtol * fabs(zn[0]) + atol;
+      if (__ballot(act && !(tol > 0.0)) != 0ull) {
+        status = HPO_ILL_INPUT;
+        yv = zn[0];
+        return ACT_DONE;
+      }
+      ewt = act ? 1.0 / tol : 0.0;
+    }
+    if (A.maxsteps > 0 && nstloc >= A.maxsteps) {
+      status = HPO_TOO_MUCH_WORK;
+      yv = zn[0];
+      return ACT_DONE;
+    }
+    if (UROUND * wrms(zn[0], ewt, n) > 1.0) {
+      status = HPO_TOO_MUCH_ACC;
+      yv = zn[0];
+      return ACT_DONE;
+    }
+    S.saved_t = S.tn;
+    ncf = 0;
+    nef = 0;
+    nflag = FIRST_CALL;
+    if (nst > 0 && S.hprime != S.h) {
+      if (qprime != q) {
+        adjust_order(qprime - q);
+        q = qprime;
+        L = q + 1;
+        qwait = L;
+      }
+      rescale();
+    }
+    return ACT_ATTEMPT;
+  };
+  // Nonlinear-solver failure (cvHandleNFlag with a recoverable flag)
+  auto nls_fail = [&]() -> int {
+    ncfn++;
+    restore();
+    ncf++;
+    S.etamax = 1.0;
+    if (ncf == MXNCF) {
+      status = HPO_CONV_FAILURE;
+      yv = zn[0];
+      return ACT_DONE;
+    }
+    S.eta = ETACF;
+    nflag = PREV_CONV_FAIL;
+    rescale();
+    return ACT_ATTEMPT;
+  };
+  // Newton converged: error test, then complete the step or retry
+  auto nls_success = [&]() -> int {
+    double dsm = S.acnrm * S.tq[2];
+    if (dsm > 1.0) {
+      nef++;
+      netf++;
+      nflag = PREV_ERR_FAIL;
+      restore();
+      if (nef == MXNEF) {
+        status = HPO_ERR_FAILURE;
+        yv = zn[0];
+        return ACT_DONE;
+      }
+      S.etamax = 1.0;
+      if (nef <= MXNEF1) {
+        double eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+        eta = fmax(ETAMIN, eta);
+        if (nef >= SMALL_NEF) eta = fmin(eta, ETAMXF);
+        S.eta = eta;
+        rescale();
+        return ACT_ATTEMPT;
+      }
+      if (q > 1) {
+        S.eta = ETAMIN;
+        adjust_order(-1);
+        L = q;
+        q--;
+        qwait = L;
+        rescale();
+        return ACT_ATTEMPT;
+      }
+      S.eta = ETAMIN;
+      S.h *= ETAMIN;
+      S.hscale = S.h;
+      qwait = LONG_WAIT;
+      yv = zn[0];
+      return ACT_EVAL;  // PH_RESTART: needs f(tn, zn[0])
+    }
+    // cvCompleteStep
+    nst++;
+    const double h = S.h;
+    S.hu = h;
+    for (int i = q; i >= 2; --i) S.tau[i] = S.tau[i - 1];
+    if (q == 1 && nst > 1) S.tau[2] = S.tau[1];
+    S.tau[1] = h;
+#pragma unroll
+    for (int j = 0; j <= QMAX; ++j)
+      if (j <= q) zn[j] += S.l[j] * acor;
+    qwait--;
+    if (qwait == 1 && q != QMAX) {
+      zn[QMAX] = acor;
+      S.saved_tq5 = S.tq[5];
+    }
+    // cvPrepareNextStep
+    if (S.etamax == 1.0) {
+      qwait = qwait > 2 ? qwait : 2;
+      qprime = q;
+      S.hprime = h;
+      S.eta = 1.0;
+    } else {
+      double etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+      double eta;
+      if (qwait != 0) {
+        eta = etaq;
+        qprime = q;
+      } else {
+        qwait = 2;
+        double etaqm1 = 0.0, etaqp1 = 0.0;
+        if (q > 1) {
+          double ddn = wrms(pick(zn, q), ewt, n) * S.tq[1];
+          etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / q) + ADDON);
+        }
+        if (q != QMAX && S.saved_tq5 != 0.0) {
+          double rr = h / S.tau[2], pw = 1.0;
+          for (int i = 0; i < L; ++i) pw *= rr;
+          double cquot = (S.tq[5] / S.saved_tq5) * pw;
+          double tempv = -cquot * zn[QMAX] + acor;
+          double dup = wrms(tempv, ewt, n) * S.tq[3];
+          etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+        }
+        double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+        if (etam < THRESH) {
+          eta = 1.0;
+          qprime = q;
+        } else if (etam == etaqm1) {
+          eta = etaqm1;
+          qprime = q - 1;
+        } else if (etam == etaq) {
+          eta = etaq;
+          qprime = q;
+        } else {
+          eta = etaqp1;
+          qprime = q + 1;
+          zn[QMAX] = acor;
+        }
+      }
+      // cvSetEta
+      if (eta < THRESH) {
+        eta = 1.0;
+        S.hprime = h;
+      } else {
+        eta = fmin(eta, S.etamax);
+        S.hprime = h * eta;
+      }
+      S.eta = eta;
+    }
+    S.etamax = (nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
+    acor *= S.tq[2];
+    nstloc++;
+    if ((S.tn - tout) * h >= 0.0) {
+      // CVodeGetDky(tout, 0)
+      double s = (tout - S.tn) / h;
+      double cj = 1.0;
+      for (int i = 0; i < q; ++i) cj *= s;
+      double yo = cj * pick(zn, q);
+#pragma unroll
+      for (int j = QMAX - 1; j >= 0; --j) {
+        if (j < q) {
+          double c = 1.0;
+          for (int i = 0; i < j; ++i) c *= s;
+          yo += c * zn[j];
+        }
+      }
+      yv = yo;
+      status = HPO_SUCCESS;
+      return ACT_DONE;
+    }
+    return ACT_BEGIN;
+  };
+
+  int phase = PH_F0;
+  int action = (status == HPO_SUCCESS) ? ACT_EVAL : ACT_DONE;
+  while (action != ACT_DONE) {
+    // ---- the single RHS site ----
+    double f = rhs(t, w, yv, P, lane, n);
+    nfe++;
+    action = ACT_DONE;
+    if (phase == PH_NEWTON) {
+      double delta = S.rl1 * zn[1] + acor - S.gamma * f;
+      int fail = 0;
+      if (need_setup) {
+        need_setup = 0;
+        if (jbad) convfail = FAIL_BAD_J;
+        // cvLsSetup / cvLsLinSys
+        const double gamma = S.gamma;
+        double dgamma = fabs(gamma / S.gammap - 1.0);
+        bool jnew = (nst == 0) || (nst >= nstlj + MSBJ) || (convfail == FAIL_BAD_J && dgamma < DGMAX_JBAD) ||
+                    (convfail == FAIL_OTHER);
+        if constexpr (LU32) {
+          if (jnew) {
+            jacobian<NS>(t, w, g, yv, P, lane);
+            nje++;
+            nstlj = nst;
+          }
+#pragma unroll
+          for (int j = 0; j < NS; ++j) {
+            double Jv = g.J[j * 64 + lane];
+            lu.J[j] = static_cast<float>(Jv);
+            lu.L.a[j] = static_cast<float>(-gamma * Jv + ((j == lane) ? 1.0 : 0.0));
+          }
+        } else {
+          if (jnew) {
+            jacobian<NS>(t, w, g, yv, P, lane);
+            nje++;
+            nstlj = nst;
+          }
+#pragma unroll
+          for (int j = 0; j < NS; ++j) lu.L.a[j] = -gamma * g.J[j * 64 + lane] + ((j == lane) ? 1.0 : 0.0);
+        }
+        jcur = jnew ? 1 : 0;
+        bool ok = lu_factor<NS>(lu.L, lane);
+        S.gamma_lu = gamma;
+        nsetups++;
+        S.gamrat = 1.0;
+        S.gammap = gamma;
+        S.crate = 1.0;
+        nstlp = nst;
+        if (!ok) fail = 1;  // singular: recoverable failure returned to the step
+      }
+      if (fail) {
+        action = nls_fail();
+      } else {
+        nni++;
+        double b = -delta;
+        double x;
+        if constexpr (LU32) {
+          float x32 = lu_solve<NS>(lu.L, static_cast<float>(b), lane);
+          x = static_cast<double>(x32);
+          const double gl = S.gamma_lu;
+          for (int sw = 0; sw < A.refine_sweeps; ++sw) {
+            // r = b - (I - gamma J) x in FP64 arithmetic
+            double Jx = 0.0;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) Jx = fma(static_cast<double>(lu.J[j]), rl(x, j), Jx);
+            double r = b - x + gl * Jx;
+            float d32 = lu_solve<NS>(lu.L, static_cast<float>(r), lane);
+            x += static_cast<double>(d32);
+          }
+        } else {
+          x = lu_solve<NS>(lu.L, b, lane);
+        }
+        if (!act) x = 0.0;
+        const double gamrat = S.gamrat;
+        if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
+        delta = x;
+        acor += delta;
+        double del = wrms(delta, ewt, n);
+        double crate = S.crate;
+        if (m > 0) {
+          crate = fmax(CRDOWN * crate, del / S.delp);
+          S.crate = crate;
+        }
+        double dcon = del * fmin(1.0, crate) / S.tq[4];
+        if (dcon <= 1.0) {
+          S.acnrm = (m == 0) ? del : wrms(acor, ewt, n);
+          jcur = 0;
+          action = nls_success();
+          if (action == ACT_EVAL) phase = PH_RESTART;
+        } else {
+          bool conv_fail = (m >= 1 && del > RDIV * S.delp);
+          if (!conv_fail) {
+            S.delp = del;
+            m++;
+            conv_fail = (m >= NLS_MAXCOR);
+          }
+          if (!conv_fail) {
+            yv = zn[0] + acor;
+            action = ACT_EVAL;  // next Newton iterate
+          } else if (!jcur) {
+            need_setup = 1;
+            jbad = 1;
+            m = 0;
+            acor = 0.0;
+            yv = zn[0];
+            action = ACT_EVAL;
+          } else {
+            action = nls_fail();
+          }
+        }
+      }
+    } else if (phase == PH_RESTART) {
+      zn[1] = S.h * f;
+      action = ACT_ATTEMPT;
+    } else if (phase == PH_F0) {
+      zn[1] = f;
+      // cvHin
+      const double tn = S.tn;
+      double tdiff = tout - tn;
+      if (tdiff == 0.0) {
+        status = HPO_TOO_CLOSE;
+        action = ACT_DONE;
+      } else {
+        double sign = tdiff > 0.0 ? 1.0 : -1.0;
+        S.hin_sign = sign;
+        double tdist = fabs(tdiff);
+        double tround = UROUND * fmax(fabs(tn), fabs(tout));
+        if (tdist < 2.0 * tround) {
+          status = HPO_TOO_CLOSE;
+          action = ACT_DONE;
+        } else {
+          double hlb = HLB_FACTOR * tround;
+          double t1 = HUB_FACTOR * fabs(zn[0]) + (rtol * fabs(zn[0]) + atol);
+          double hub_inv = wmax(act ? fabs(zn[1]) / t1 : 0.0);
+          double hub = HUB_FACTOR * tdist;
+          if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+          double hg = sqrt(hlb * hub);
+          S.hlb = hlb;
+          S.hub = hub;
+          S.hg = hg;
+          if (hub < hlb) {
+            S.h = sign * hg;
+            S.hscale = S.h;
+            S.hprime = S.h;
+            zn[1] *= S.h;
+            action = ACT_BEGIN;
+          } else {
+            count1 = 1;
+            yv = (hg * sign) * zn[1] + zn[0];
+            phase = PH_HIN;
+            action = ACT_EVAL;
+          }
+        }
+      }
+    } else {  // PH_HIN: cvYddNorm result
+      const double hg = S.hg, hub = S.hub, sign = S.hin_sign;
+      double hgs = hg * sign;
+      double tempv = (f - zn[1]) * (1.0 / hgs);
+      double yddnrm = wrms(tempv, ewt, n);
+      double hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+      bool fin = false;
+      if (count1 == MAX_ITERS) {
+        fin = true;
+      } else {
+        double hrat = hnew / hg;
+        if (hrat > 0.5 && hrat < 2.0) {
+          fin = true;
+        } else if (count1 > 1 && hrat > 2.0) {
+          hnew = hg;
+          fin = true;
+        }
+      }
+      if (!fin) {
+        S.hg = hnew;
+        count1++;
+        yv = (hnew * sign) * zn[1] + zn[0];
+        action = ACT_EVAL;
+      } else {
+        double h0 = H_BIAS * hnew;
+        if (h0 < S.hlb) h0 = S.hlb;
+        if (h0 > hub) h0 = hub;
+        S.h = sign * h0;
+        S.hscale = S.h;
+        S.hprime = S.h;
+        zn[1] *= S.h;
+        action = ACT_BEGIN;
+      }
+    }
+    // run the step bookkeeping until the next RHS evaluation or the end
+    while (action == ACT_BEGIN || action == ACT_ATTEMPT) {
+      if (action == ACT_BEGIN) action = begin();
+      if (action == ACT_ATTEMPT) {
+        action = attempt();
+        phase = PH_NEWTON;
+      }
+    }
+  }
+  if (act) A.y_out[cell * A.ld + lane] = yv;
+  if (lane == 0) {
+    A.status[cell] = status;
+    if (A.stats) {
+      int* st = A.stats + cell * HPO_NSTATS;
+      st[0] = nst;
+      st[1] = nfe;
+      st[2] = nje;
+      st[3] = nsetups;
+      st[4] = nni;
+      st[5] = ncfn;
+      st[6] = netf;
+      st[7] = q;
+    }
+  }
+  wave_sync();
+}
+
+template <int NS, bool LU32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void bdf_kernel(BdfArgs A) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const DevMechLayout& Lay = A.lay;
+  for (int i = threadIdx.x * 16; i < Lay.lds_bytes; i += blockDim.x * 16)
+    *reinterpret_cast<uint4*>(smem + i) = *reinterpret_cast<const uint4*>(A.mech + i);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  Tabs t = make_tabs(smem, A.mech, Lay);
+  const int wl_doubles = wave_lds_doubles(Lay.R, Lay.n_partials) + kScalDoubles;
+  double* wbase = reinterpret_cast<double*>(smem + Lay.lds_bytes) + wave * wl_doubles;
+  WaveLds w = make_wave_lds(wbase, Lay.R, Lay.n_partials);
+  Scal& S = *reinterpret_cast<Scal*>(wbase + wave_lds_doubles(Lay.R, Lay.n_partials));
+  const int64_t slot = static_cast<int64_t>(blockIdx.x) * wpb + wave;
+  WaveScratch g = make_wave_scratch(A.scratch + slot * A.scratch_stride, Lay);
+  if (lane < 16) g.coefbuf[lane] = t.coef[lane];
+  for (;;) {
+    unsigned int c = 0;
+    if (lane == 0) c = atomicAdd(A.queue, 1u);
+    c = __builtin_amdgcn_readfirstlane(c);
+    if (static_cast<int64_t>(c) >= A.ncells) break;
+    int64_t cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
+    integrate_cell<NS, LU32>(A, t, w, g, S, cell, lane);
+  }
+}
+
+// ---- component kernels used by the parity tests ----
+template <int NS, int MODE>
+__global__ __launch_bounds__(256) void eval_kernel(EvalArgs A) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const DevMechLayout& Lay = A.lay;
+  for (int i = threadIdx.x * 16; i < Lay.lds_bytes; i += blockDim.x * 16)
+    *reinterpret_cast<uint4*>(smem + i) = *reinterpret_cast<const uint4*>(A.mech + i);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  Tabs t = make_tabs(smem, A.mech, Lay);
+  const int wl_doubles = wave_lds_doubles(Lay.R, Lay.n_partials);
+  WaveLds w = make_wave_lds(reinterpret_cast<double*>(smem + Lay.lds_bytes) + wave * wl_doubles, Lay.R, Lay.n_partials);
+  const int64_t slot = static_cast<int64_t>(blockIdx.x) * wpb + wave;
+  WaveScratch g = make_wave_scratch(A.scratch + slot * A.scratch_stride, Lay);
+  if (lane < 16) g.coefbuf[lane] = t.coef[lane];
+  for (int64_t c = slot; c < A.ncells; c += static_cast<int64_t>(gridDim.x) * wpb) {
+    double y = lane < NS ? A.y[c * A.ld + lane] : 0.0;
+    double P = A.P[c];
+    if constexpr (MODE == 0) {
+      double f = rhs(t, w, y, P, lane, NS);
+      if (lane < NS) A.out[c * A.ld + lane] = f;
+    } else {
+      jacobian<NS>(t, w, g, y, P, lane);
+      if (lane < NS)
+        for (int j = 0; j < NS; ++j) A.out[(c * NS + lane) * NS + j] = g.J[j * 64 + lane];
+    }
+  }
+}
+
+template <int NS, typename F>
+__global__ __launch_bounds__(64) void lu_kernel(LuArgs A) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = blockIdx.x;
+  LuReg<NS, F> L;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) L.a[j] = lane < NS ? static_cast<F>(A.M[(c * NS + lane) * NS + j]) : F(0);
+  bool ok = lu_factor<NS>(L, lane);
+  F b = lane < NS ? static_cast<F>(A.b[c * NS + lane]) : F(0);
+  F x = lu_solve<NS>(L, b, lane);
+  if (lane < NS) A.x[c * NS + lane] = static_cast<double>(x);
+  if (lane == 0) A.ok[c] = ok ? 1 : 0;
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host)
+// ---------------------------------------------------------------------------
+template <int NS>
+static hipError_t launch_bdf_ns(const BdfArgs& A, int lu32, int grid, int block, size_t shmem, hipStream_t st) {
+  if (lu32)
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, true>), dim3(grid), dim3(block), shmem, st, A);
+  else
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, false>), dim3(grid), dim3(block), shmem, st, A);
+  return hipGetLastError();
+}
+
+template <int NS>
+static const void* bdf_fn(int lu32) {
+  return lu32 ? reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true>)
+              : reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false>);
+}
+
+#define HPO_FOR_EACH_NS(X) X(10) X(53)
+
+int bdf_wave_lds_doubles(const DevMechLayout& L) { return dev::wave_lds_doubles(L.R, L.n_partials) + dev::kScalDoubles; }
+int eval_wave_lds_doubles(const DevMechLayout& L) { return dev::wave_lds_doubles(L.R, L.n_partials); }
+int64_t wave_scratch_doubles_host(const DevMechLayout& L) { return dev::wave_scratch_doubles(L); }
+
+bool kernel_supports(int ns) {
+#define HPO_CASE(N) \
+  if (ns == N) return true;
+  HPO_FOR_EACH_NS(HPO_CASE)
+#undef HPO_CASE
+  return false;
+}
+
+const void* bdf_kernel_fn(int ns, int lu32) {
+#define HPO_CASE(N) \
+  if (ns == N) return bdf_fn<N>(lu32);
+  HPO_FOR_EACH_NS(HPO_CASE)
+#undef HPO_CASE
+  return nullptr;
+}
+
+hipError_t launch_bdf(int ns, const BdfArgs& A, int lu32, int grid, int block, size_t shmem, hipStream_t st) {
+#define HPO_CASE(N) \
+  if (ns == N) return launch_bdf_ns<N>(A, lu32, grid, block, shmem, st);
+  HPO_FOR_EACH_NS(HPO_CASE)
+#undef HPO_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_eval(int ns, const EvalArgs& A, int grid, int block, size_t shmem, hipStream_t st) {
+#define HPO_CASE(N)                                                                         \
+  if (ns == N) {                                                                            \
+    if (A.mode == 0)                                                                        \
+      hipLaunchKernelGGL((dev::eval_kernel<N, 0>), dim3(grid), dim3(block), shmem, st, A);  \
+    else                                                                                    \
+      hipLaunchKernelGGL((dev::eval_kernel<N, 1>), dim3(grid), dim3(block), shmem, st, A);  \
+    return hipGetLastError();                                                               \
+  }
+  HPO_FOR_EACH_NS(HPO_CASE)
+#undef HPO_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_lu(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st) {
+#define HPO_CASE(N)                                                                                        \
+  if (ns == N) {                                                                                           \
+    if (fp32)                                                                                              \
+      hipLaunchKernelGGL((dev::lu_kernel<N, float>), dim3(static_cast<unsigned>(ncases)), dim3(64), 0, st, A);  \
+    else                                                                                                   \
+      hipLaunchKernelGGL((dev::lu_kernel<N, double>), dim3(static_cast<unsigned>(ncases)), dim3(64), 0, st, A); \
+    return hipGetLastError();                                                                              \
+  }
+  HPO_FOR_EACH_NS(HPO_CASE)
+#undef HPO_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace hpo

@@ -1,0 +1,71 @@
+// Kernel argument blocks and host launch entry points of the HIP path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mechanism.hpp"
+
+// per-cell status codes, CVODE's return-code values (cvode.h)
+#define HPO_SUCCESS 0
+#define HPO_TOO_MUCH_WORK (-1)
+#define HPO_TOO_MUCH_ACC (-2)
+#define HPO_ERR_FAILURE (-3)
+#define HPO_CONV_FAILURE (-4)
+#define HPO_ILL_INPUT (-22)
+#define HPO_TOO_CLOSE (-27)
+
+#define HPO_NSTATS 8  // nst, nfe, nje, nsetups, nni, ncfn, netf, q_last
+
+namespace hpo {
+
+struct BdfArgs {
+  const uint8_t* mech;       // device table blob
+  DevMechLayout lay;
+  const double* y_in;        // [ncells][ld] state rows y = [T, Y_0..Y_{K-2}]
+  double* y_out;             // [ncells][ld]
+  const double* P;           // [ncells] Pa
+  const double* dt;          // [ncells] s
+  int32_t* status;           // [ncells]
+  int32_t* stats;            // [ncells][HPO_NSTATS] or null
+  const int32_t* order;      // work-queue order of cells, or null for 0..ncells-1
+  uint32_t* queue;           // work-queue counter (zero before launch)
+  double* scratch;           // per-wave scratch
+  int64_t scratch_stride;    // doubles per wave slot
+  int64_t ncells;
+  int64_t ld;
+  double rtol, atol;
+  int32_t maxsteps;
+  int32_t refine_sweeps;
+};
+
+struct EvalArgs {
+  const uint8_t* mech;
+  DevMechLayout lay;
+  const double* y;           // [ncells][ld]
+  const double* P;           // [ncells]
+  double* out;               // mode 0: [ncells][ld] f; mode 1: [ncells][NS][NS] row-major J
+  double* scratch;
+  int64_t scratch_stride;
+  int64_t ncells;
+  int64_t ld;
+  int32_t mode;
+};
+
+struct LuArgs {
+  const double* M;           // [ncases][NS][NS] row-major
+  const double* b;           // [ncases][NS]
+  double* x;                 // [ncases][NS]
+  int32_t* ok;               // [ncases]
+};
+
+bool kernel_supports(int ns);
+int bdf_wave_lds_doubles(const DevMechLayout& L);
+int eval_wave_lds_doubles(const DevMechLayout& L);
+int64_t wave_scratch_doubles_host(const DevMechLayout& L);
+const void* bdf_kernel_fn(int ns, int lu32);
+hipError_t launch_bdf(int ns, const BdfArgs& A, int lu32, int grid, int block, size_t shmem, hipStream_t st);
+hipError_t launch_eval(int ns, const EvalArgs& A, int grid, int block, size_t shmem, hipStream_t st);
+hipError_t launch_lu(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st);
+
+}  // namespace hpo

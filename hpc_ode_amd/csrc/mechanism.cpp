@@ -1,0 +1,521 @@
+#include "mechanism.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <regex>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+
+#include "yaml_lite.hpp"
+
+namespace hpo {
+
+namespace {
+
+// Atomic weights as Cantera 2.x tabulates them (kg/kmol); pyJac's molecular weights
+// (e.g. 2.016 for H2, 39.95 for AR in dydt.c:44-132) follow from these.
+double atomic_weight(const std::string& e) {
+  static const std::map<std::string, double> aw = {
+      {"H", 1.008},   {"D", 2.014102}, {"He", 4.002602},      {"C", 12.011},  {"N", 14.007},
+      {"O", 15.999},  {"F", 18.998403163}, {"Ne", 20.1797},    {"Si", 28.085}, {"S", 32.06},
+      {"Cl", 35.45},  {"Ar", 39.95},   {"Kr", 83.798},        {"Xe", 131.293}, {"E", 0.000548579909}};
+  auto it = aw.find(e);
+  if (it == aw.end()) throw std::runtime_error("unknown element '" + e + "'");
+  return it->second;
+}
+
+std::string trim(const std::string& s) {
+  size_t b = s.find_first_not_of(" \t");
+  if (b == std::string::npos) return "";
+  size_t e = s.find_last_not_of(" \t");
+  return s.substr(b, e - b + 1);
+}
+
+struct Side {
+  std::vector<std::pair<std::string, double>> terms;
+  std::string collider;  // "(+X)" falloff collider, empty if none
+  bool plus_m = false;
+};
+
+Side parse_side(std::string s) {
+  Side out;
+  size_t lp = s.find("(+");
+  if (lp != std::string::npos) {
+    size_t rp = s.find(')', lp);
+    if (rp == std::string::npos) throw std::runtime_error("bad falloff collider in '" + s + "'");
+    out.collider = trim(s.substr(lp + 2, rp - lp - 2));
+    s = s.substr(0, lp) + s.substr(rp + 1);
+  }
+  size_t pos = 0;
+  while (pos <= s.size()) {
+    size_t nx = s.find(" + ", pos);
+    std::string tok = trim(s.substr(pos, nx == std::string::npos ? std::string::npos : nx - pos));
+    if (!tok.empty()) {
+      double coef = 1.0;
+      std::string name = tok;
+      size_t sp = tok.find(' ');
+      if (sp != std::string::npos) {
+        std::string c = tok.substr(0, sp);
+        char* end = nullptr;
+        double v = std::strtod(c.c_str(), &end);
+        if (end && *end == '\0') {
+          coef = v;
+          name = trim(tok.substr(sp + 1));
+        }
+      }
+      if (name == "M") out.plus_m = true;
+      else out.terms.emplace_back(name, coef);
+    }
+    if (nx == std::string::npos) break;
+    pos = nx + 3;
+  }
+  return out;
+}
+
+std::vector<std::pair<int, double>> merge(const std::vector<std::pair<std::string, double>>& t,
+                                          const std::map<std::string, int>& index) {
+  std::map<int, double> acc;
+  for (auto& kv : t) {
+    auto it = index.find(kv.first);
+    if (it == index.end()) throw std::runtime_error("unknown species '" + kv.first + "' in reaction");
+    acc[it->second] += kv.second;
+  }
+  return {acc.begin(), acc.end()};
+}
+
+double unit_factor_conc(const YNode& units) {
+  // (length^3 / quantity) -> m^3/kmol
+  double lf = 1.0, qf = 1.0;
+  if (const YNode* l = units.find("length")) {
+    const std::string& u = l->as_string();
+    lf = u == "cm" ? 1e-2 : u == "mm" ? 1e-3 : u == "m" ? 1.0 : throw std::runtime_error("length unit " + u);
+  }
+  if (const YNode* q = units.find("quantity")) {
+    const std::string& u = q->as_string();
+    qf = u == "mol" ? 1e-3 : u == "kmol" ? 1.0 : u == "molec" ? 1.0 / 6.02214076e26
+                                                : throw std::runtime_error("quantity unit " + u);
+  }
+  return lf * lf * lf / qf;
+}
+
+double unit_factor_ea(const YNode& units) {
+  // J/kmol per activation-energy unit
+  const YNode* a = units.find("activation-energy");
+  if (!a) return 1.0;
+  const std::string& u = a->as_string();
+  if (u == "cal/mol") return 4184.0;
+  if (u == "kcal/mol") return 4184.0e3;
+  if (u == "J/mol") return 1e3;
+  if (u == "kJ/mol") return 1e6;
+  if (u == "J/kmol") return 1.0;
+  if (u == "K") return kRGas;
+  throw std::runtime_error("activation-energy unit " + u);
+}
+
+void arrhenius(const YNode& rc, double order, double cf, double ef, double& lnA, double& b, double& th,
+               double& A) {
+  A = rc["A"].as_double() * std::pow(cf, order - 1.0);
+  b = rc["b"].as_double();
+  th = rc["Ea"].as_double() * ef / kRGas;
+  lnA = A > 0 ? std::log(A) : -INFINITY;
+}
+
+}  // namespace
+
+Mechanism load_mechanism_yaml(const std::string& path) {
+  YNode doc = yaml_load_file(path);
+  Mechanism m;
+  m.path = path;
+  YNode nounits;
+  nounits.kind = YNode::Map;
+  const YNode& units = doc.has("units") ? doc["units"] : nounits;
+  const YNode& phase = doc["phases"][0];
+  const YNode& spl = phase["species"];
+  std::map<std::string, int> index;
+  for (size_t i = 0; i < spl.size(); ++i) {
+    m.species.push_back(spl[i].as_string());
+    index[spl[i].as_string()] = static_cast<int>(i);
+  }
+  const int K = m.K();
+  m.W.assign(K, 0.0);
+  m.tmid.assign(K, 0.0);
+  m.nasa.assign(static_cast<size_t>(K) * 14, 0.0);
+  const YNode& sdb = doc["species"];
+  std::map<std::string, const YNode*> byname;
+  for (size_t i = 0; i < sdb.size(); ++i) byname[sdb[i]["name"].as_string()] = &sdb[i];
+  for (int k = 0; k < K; ++k) {
+    auto it = byname.find(m.species[k]);
+    if (it == byname.end()) throw std::runtime_error("species '" + m.species[k] + "' has no entry");
+    const YNode& s = *it->second;
+    double w = 0;
+    for (auto& kv : s["composition"].map) w += atomic_weight(kv.first) * kv.second.as_double();
+    m.W[k] = w;
+    const YNode& th = s["thermo"];
+    if (th["model"].as_string() != "NASA7")
+      throw std::runtime_error(m.species[k] + ": thermo model " + th["model"].as_string() + " unsupported");
+    const YNode& tr = th["temperature-ranges"];
+    const YNode& data = th["data"];
+    const YNode& lo = data[0];
+    const YNode& hi = data.size() > 1 ? data[1] : data[0];
+    m.tmid[k] = data.size() > 1 ? tr[1].as_double() : tr[tr.size() - 1].as_double();
+    for (int j = 0; j < 7; ++j) {
+      m.nasa[14 * k + j] = lo[j].as_double();
+      m.nasa[14 * k + 7 + j] = hi[j].as_double();
+    }
+  }
+  const double cf = unit_factor_conc(units), ef = unit_factor_ea(units);
+  const YNode& rl = doc["reactions"];
+  for (size_t ri = 0; ri < rl.size(); ++ri) {
+    const YNode& rd = rl[ri];
+    Reaction rx;
+    rx.equation = rd["equation"].as_string();
+    std::string eq = rx.equation, lhs, rhs;
+    size_t p;
+    if ((p = eq.find("<=>")) != std::string::npos) {
+      lhs = eq.substr(0, p);
+      rhs = eq.substr(p + 3);
+    } else if ((p = eq.find("=>")) != std::string::npos) {
+      lhs = eq.substr(0, p);
+      rhs = eq.substr(p + 2);
+      rx.reversible = false;
+    } else if ((p = eq.find('=')) != std::string::npos) {
+      lhs = eq.substr(0, p);
+      rhs = eq.substr(p + 1);
+    } else {
+      throw std::runtime_error("bad equation '" + eq + "'");
+    }
+    if (rd.has("orders")) throw std::runtime_error("non-standard reaction orders unsupported: " + eq);
+    Side L = parse_side(lhs), Rr = parse_side(rhs);
+    rx.reactants = merge(L.terms, index);
+    rx.products = merge(Rr.terms, index);
+    double order = 0;
+    for (auto& kv : rx.reactants) order += kv.second;
+    std::string type = rd.has("type") ? rd["type"].as_string() : std::string("elementary");
+    std::vector<std::pair<int, double>> effs;
+    if (const YNode* e = rd.find("efficiencies"))
+      for (auto& kv : e->map) {
+        auto it = index.find(kv.first);
+        if (it != index.end()) effs.emplace_back(it->second, kv.second.as_double());
+      }
+    std::sort(effs.begin(), effs.end());
+    double deff = rd.has("default-efficiency") ? rd["default-efficiency"].as_double() : 1.0;
+    auto set_eff = [&](double base, const std::vector<std::pair<int, double>>& e) {
+      rx.eff_base = base;
+      rx.eff.clear();
+      for (auto& kv : e) rx.eff.emplace_back(kv.first, kv.second - base);
+    };
+    if (type == "three-body" || (type == "elementary" && L.plus_m)) {
+      rx.kind = kThreeBody;
+      arrhenius(rd["rate-constant"], order + 1, cf, ef, rx.lnA, rx.b, rx.theta, rx.A);
+      set_eff(deff, effs);
+    } else if (type == "falloff") {
+      arrhenius(rd["high-P-rate-constant"], order, cf, ef, rx.lnA, rx.b, rx.theta, rx.A);
+      arrhenius(rd["low-P-rate-constant"], order + 1, cf, ef, rx.lnA0, rx.b0, rx.theta0, rx.A0);
+      if (!L.collider.empty() && L.collider != "M") {
+        auto it = index.find(L.collider);
+        if (it == index.end()) throw std::runtime_error("unknown collider " + L.collider);
+        set_eff(0.0, {{it->second, 1.0}});
+      } else {
+        set_eff(deff, effs);
+      }
+      if (const YNode* t = rd.find("Troe")) {
+        rx.kind = kTroe;
+        rx.troe[0] = (*t)["A"].as_double();
+        rx.troe[1] = (*t)["T3"].as_double();
+        rx.troe[2] = (*t)["T1"].as_double();
+        rx.troe_has_t2 = t->has("T2");
+        rx.troe[3] = rx.troe_has_t2 ? (*t)["T2"].as_double() : 0.0;
+      } else if (rd.has("SRI")) {
+        throw std::runtime_error("SRI falloff unsupported: " + eq);
+      } else {
+        rx.kind = kLindemann;
+      }
+    } else if (type == "elementary") {
+      rx.kind = kElementary;
+      arrhenius(rd["rate-constant"], order, cf, ef, rx.lnA, rx.b, rx.theta, rx.A);
+    } else {
+      throw std::runtime_error("reaction type '" + type + "' unsupported: " + eq);
+    }
+    m.reactions.push_back(std::move(rx));
+  }
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Device tables
+// ---------------------------------------------------------------------------
+namespace {
+
+int align_to(int x, int a) { return (x + a - 1) / a * a; }
+
+int coef_index(std::vector<double>& tab, double v) {
+  for (size_t i = 0; i < tab.size(); ++i)
+    if (tab[i] == v) return static_cast<int>(i);
+  if (tab.size() >= 16) throw std::runtime_error("more than 15 distinct stoichiometric coefficients");
+  tab.push_back(v);
+  return static_cast<int>(tab.size() - 1);
+}
+
+}  // namespace
+
+DevMechBlob build_device_tables(const Mechanism& m) {
+  const int K = m.K(), R = m.R();
+  if (K < 2 || K > 63) throw std::runtime_error("device path supports 2..63 species");
+  if (R > 4095) throw std::runtime_error("device path supports at most 4095 reactions");
+  DevMechBlob blob;
+  DevMechLayout& L = blob.lay;
+  L.K = K;
+  L.R = R;
+
+  // Device reaction order: Troe, Lindemann, three-body, reversible, irreversible, so that
+  // the 64-reaction chunks a wave evaluates together mostly take the same branch. The
+  // order is stable within a group; results do not depend on it.
+  std::vector<const Reaction*> rx_of(R);
+  {
+    std::vector<int> idx(R);
+    for (int r = 0; r < R; ++r) idx[r] = r;
+    auto group = [&](int r) {
+      const Reaction& x = m.reactions[r];
+      if (x.kind == kTroe) return 0;
+      if (x.kind == kLindemann) return 1;
+      if (x.kind == kThreeBody) return 2;
+      return x.reversible ? 3 : 4;
+    };
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return group(a) < group(b); });
+    for (int d = 0; d < R; ++d) rx_of[d] = &m.reactions[idx[d]];
+  }
+
+  // coefficient table: entry 0 is 0.0
+  std::vector<double> coef{0.0};
+
+  // net stoichiometry per reaction
+  std::vector<std::map<int, double>> net(R);
+  int n_pdep = 0, n_eff = 0;
+  for (int r = 0; r < R; ++r) {
+    const Reaction& rx = *rx_of[r];
+    if (static_cast<int>(rx.reactants.size()) > kMaxSideSpecies ||
+        static_cast<int>(rx.products.size()) > kMaxSideSpecies)
+      throw std::runtime_error("more than 3 species on a reaction side: " + rx.equation);
+    for (auto& kv : rx.reactants) {
+      if (kv.second != std::floor(kv.second) || kv.second < 1 || kv.second > 3)
+        throw std::runtime_error("non-integer stoichiometry unsupported: " + rx.equation);
+      net[r][kv.first] -= kv.second;
+    }
+    for (auto& kv : rx.products) {
+      if (kv.second != std::floor(kv.second) || kv.second < 1 || kv.second > 3)
+        throw std::runtime_error("non-integer stoichiometry unsupported: " + rx.equation);
+      net[r][kv.first] += kv.second;
+    }
+    if (rx.kind != kElementary) {
+      ++n_pdep;
+      n_eff += static_cast<int>(rx.eff.size());
+    }
+  }
+  L.n_pdep = n_pdep;
+  L.n_eff = n_eff;
+
+  // balanced species gather over the nonzeros of the net stoichiometric matrix
+  struct Nz {
+    int k, r, c;
+  };
+  std::vector<Nz> nz;
+  for (int k = 0; k < K; ++k)
+    for (int r = 0; r < R; ++r) {
+      auto it = net[r].find(k);
+      if (it != net[r].end() && it->second != 0.0) nz.push_back({k, r, coef_index(coef, it->second)});
+    }
+  const int NNZ = static_cast<int>(nz.size());
+  const int per_lane = std::max(1, (NNZ + 63) / 64);
+  L.n_gather_lane = per_lane;
+  std::vector<uint32_t> gather(static_cast<size_t>(per_lane) * 64, 0u);
+  std::vector<int> part_species;  // species of each partial slot
+  for (int lane = 0; lane < 64; ++lane) {
+    int b = lane * per_lane, e = std::min(NNZ, b + per_lane);
+    for (int t = b; t < e; ++t) {
+      bool flush = (t + 1 == e) || (nz[t + 1].k != nz[t].k);
+      uint32_t part = 0;
+      if (flush) {
+        part = static_cast<uint32_t>(part_species.size());
+        part_species.push_back(nz[t].k);
+      }
+      uint32_t w = static_cast<uint32_t>(nz[t].r) | (static_cast<uint32_t>(nz[t].c) << 12) |
+                   (flush ? (1u << 16) : 0u) | (part << 17);
+      gather[static_cast<size_t>(t - b) * 64 + lane] = w;
+    }
+    // padding entries: r=0, coef 0 (0.0), no flush -> contribute nothing
+  }
+  L.n_partials = static_cast<int>(part_species.size());
+  if (L.n_partials > 32767) throw std::runtime_error("too many gather partials");
+  std::vector<uint16_t> part_ptr(K + 1, 0);
+  {
+    // partial slots of a species are contiguous because chunks are contiguous in species order
+    int s = 0;
+    for (int k = 0; k < K; ++k) {
+      part_ptr[k] = static_cast<uint16_t>(s);
+      while (s < L.n_partials && part_species[s] == k) ++s;
+    }
+    part_ptr[K] = static_cast<uint16_t>(s);
+    if (s != L.n_partials) throw std::runtime_error("gather partials not contiguous");
+  }
+
+  // Jacobian slots: per reaction reactant slots, product slots (reversible), efficiency slots
+  std::vector<uint16_t> slot_base(R + 1, 0);
+  std::vector<int> slot_species;
+  for (int r = 0; r < R; ++r) {
+    const Reaction& rx = *rx_of[r];
+    slot_base[r] = static_cast<uint16_t>(slot_species.size());
+    for (auto& kv : rx.reactants) slot_species.push_back(kv.first);
+    if (rx.reversible)
+      for (auto& kv : rx.products) slot_species.push_back(kv.first);
+    if (rx.kind != kElementary)
+      for (auto& kv : rx.eff) slot_species.push_back(kv.first);
+  }
+  slot_base[R] = static_cast<uint16_t>(slot_species.size());
+  L.n_slots = static_cast<int>(slot_species.size());
+  if (L.n_slots > 65535) throw std::runtime_error("too many Jacobian slots");
+  L.n_coef = static_cast<int>(coef.size());
+
+  // Jacobian terms grouped by cell (col i, row): rows 0..K-2 species, row K = temperature
+  std::map<int, std::vector<uint64_t>> cells;  // cell -> terms (src | coef<<16)
+  for (int r = 0; r < R; ++r) {
+    for (int s = slot_base[r]; s < slot_base[r + 1]; ++s) {
+      int i = slot_species[s];
+      for (auto& kv : net[r]) {
+        int k = kv.first;
+        if (kv.second == 0.0 || k == K - 1) continue;
+        int cell = i * 64 + k;
+        cells[cell].push_back(static_cast<uint64_t>(s) | (static_cast<uint64_t>(coef_index(coef, kv.second)) << 16));
+      }
+      int cell = i * 64 + K;
+      cells[cell].push_back(static_cast<uint64_t>(s) | (static_cast<uint64_t>(16 + r) << 16));
+    }
+  }
+  L.n_coef = static_cast<int>(coef.size());
+  // greedy balanced assignment of cells to lanes
+  std::vector<std::pair<int, int>> order;  // (terms, cell)
+  for (auto& kv : cells) order.emplace_back(static_cast<int>(kv.second.size()), kv.first);
+  std::sort(order.begin(), order.end(), [](auto& a, auto& b) { return a.first > b.first || (a.first == b.first && a.second < b.second); });
+  std::vector<std::vector<int>> lane_cells(64);
+  std::vector<int> load(64, 0);
+  for (auto& oc : order) {
+    int best = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+    lane_cells[best].push_back(oc.second);
+    load[best] += oc.first;
+  }
+  int maxload = *std::max_element(load.begin(), load.end());
+  L.n_jterm_lane = std::max(1, maxload);
+  const uint64_t dummy = 4095ull | (1ull << 48);  // cell (63,63) unused, coef 0 -> writes 0
+  std::vector<uint64_t> jterm(static_cast<size_t>(L.n_jterm_lane) * 64, dummy);
+  for (int lane = 0; lane < 64; ++lane) {
+    std::sort(lane_cells[lane].begin(), lane_cells[lane].end());
+    int t = 0;
+    for (int cell : lane_cells[lane]) {
+      auto& v = cells[cell];
+      for (size_t j = 0; j < v.size(); ++j) {
+        uint64_t w = static_cast<uint64_t>(cell) | (v[j] << 16) | ((j + 1 == v.size()) ? (1ull << 48) : 0ull);
+        jterm[static_cast<size_t>(t) * 64 + lane] = w;
+        ++t;
+      }
+    }
+  }
+
+  // ---- layout ----
+  int off = 0;
+  L.off_species = off;
+  off += K * kSpStride * 8;
+  off = align_to(off, 16);
+  L.off_hdr = off;
+  off += R * static_cast<int>(sizeof(RxnHdr));
+  L.off_arr = off;
+  off += R * 3 * 8;
+  L.off_pdep = off;
+  off += n_pdep * static_cast<int>(sizeof(PdepRec));
+  L.off_eff_val = off;
+  off += n_eff * 8;
+  L.off_eff_sp = off;
+  off += align_to(std::max(n_eff, 1), 8);
+  L.off_coef = off;
+  off += 16 * 8;
+  L.off_gather = off;
+  off += per_lane * 64 * 4;
+  L.off_part_ptr = off;
+  off += align_to((K + 1) * 2, 16);
+  off = align_to(off, 16);
+  L.lds_bytes = off;
+  L.off_slot_base = off;
+  off += align_to((R + 1) * 2, 16);
+  L.off_jterm = off;
+  off += L.n_jterm_lane * 64 * 8;
+  L.total_bytes = off;
+
+  blob.bytes.assign(off, 0);
+  uint8_t* B = blob.bytes.data();
+  double* sp = reinterpret_cast<double*>(B + L.off_species);
+  for (int k = 0; k < K; ++k) {
+    sp[k * kSpStride + 0] = 1.0 / m.W[k];
+    sp[k * kSpStride + 1] = m.W[k];
+    sp[k * kSpStride + 2] = m.tmid[k];
+    for (int j = 0; j < 14; ++j) sp[k * kSpStride + 3 + j] = m.nasa[14 * k + j];
+  }
+  RxnHdr* hdr = reinterpret_cast<RxnHdr*>(B + L.off_hdr);
+  double* arr = reinterpret_cast<double*>(B + L.off_arr);
+  PdepRec* pd = reinterpret_cast<PdepRec*>(B + L.off_pdep);
+  double* effv = reinterpret_cast<double*>(B + L.off_eff_val);
+  uint8_t* effs = B + L.off_eff_sp;
+  int ip = 0, ie = 0;
+  for (int r = 0; r < R; ++r) {
+    const Reaction& rx = *rx_of[r];
+    RxnHdr h;
+    std::memset(&h, 0, sizeof(h));
+    for (int s = 0; s < 6; ++s) h.sp[s] = 0xFF;
+    for (size_t s = 0; s < rx.reactants.size(); ++s) {
+      h.sp[s] = static_cast<uint8_t>(rx.reactants[s].first);
+      h.nu[s] = static_cast<uint8_t>(rx.reactants[s].second);
+    }
+    for (size_t s = 0; s < rx.products.size(); ++s) {
+      h.sp[3 + s] = static_cast<uint8_t>(rx.products[s].first);
+      h.nu[3 + s] = static_cast<uint8_t>(rx.products[s].second);
+    }
+    h.kind = static_cast<uint8_t>(rx.kind);
+    bool constk = (rx.b == 0.0 && rx.theta == 0.0);
+    h.flags = (rx.reversible ? 1 : 0) | (constk ? 2 : 0) | (rx.troe_has_t2 ? 4 : 0);
+    arr[3 * r + 0] = constk ? rx.A : rx.lnA;
+    arr[3 * r + 1] = rx.b;
+    arr[3 * r + 2] = rx.theta;
+    if (rx.kind != kElementary) {
+      h.pidx = static_cast<uint16_t>(ip);
+      PdepRec& P = pd[ip];
+      P.eff_base = rx.eff_base;
+      P.eff_begin = ie;
+      for (auto& kv : rx.eff) {
+        effv[ie] = kv.second;
+        effs[ie] = static_cast<uint8_t>(kv.first);
+        ++ie;
+      }
+      P.eff_end = ie;
+      bool constk0 = (rx.b0 == 0.0 && rx.theta0 == 0.0);
+      if (rx.kind >= kLindemann && constk0) h.flags |= 8;
+      P.lnA0 = (rx.kind >= kLindemann && constk0) ? rx.A0 : rx.lnA0;
+      P.b0 = rx.b0;
+      P.theta0 = rx.theta0;
+      P.troe_a = rx.troe[0];
+      P.troe_T3 = rx.troe[1];
+      P.troe_T1 = rx.troe[2];
+      P.troe_T2 = rx.troe[3];
+      ++ip;
+    }
+    hdr[r] = h;
+  }
+  double* ctab = reinterpret_cast<double*>(B + L.off_coef);
+  for (size_t i = 0; i < coef.size(); ++i) ctab[i] = coef[i];
+  std::memcpy(B + L.off_gather, gather.data(), gather.size() * 4);
+  std::memcpy(B + L.off_part_ptr, part_ptr.data(), part_ptr.size() * 2);
+  std::memcpy(B + L.off_slot_base, slot_base.data(), slot_base.size() * 2);
+  std::memcpy(B + L.off_jterm, jterm.data(), jterm.size() * 8);
+  return blob;
+}
+
+}  // namespace hpo

@@ -1,0 +1,108 @@
+// Chemistry mechanism: host tables read from a Cantera YAML file, and the packed
+// device tables the HIP kernels stage into LDS.
+//
+// Replaces the reference's pyJac code generation (pyJac_mechanism_generator.sh:3-4,
+// src/Mechanism/CPU/*.c, src/Mechanism/GPU/*.cu): the same physics is evaluated from
+// tables at run time, so any mechanism file in this format can be loaded without
+// regenerating code. Conventions follow the generated code:
+//   R = 8314.4621 J/(kmol K) (chem_utils.c:42), A in SI-kmol, theta = Ea/R,
+//   NASA7 low range for T <= T_mid (chem_utils.c:197), third-body [M] = base*P/(RT)
+//   + sum (eff-base) C (rxn_rates_pres_mod.c:20-23), Troe falloff
+//   (rxn_rates_pres_mod.c:65-73), the last species eliminated (mechanism.h:5).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hpo {
+
+constexpr double kRGas = 8314.4621;  // J/(kmol K)
+constexpr double kPAtm = 101325.0;   // Pa
+constexpr int kMaxSideSpecies = 3;   // species per reaction side on the device path
+
+enum RxnKind : int { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3 };
+
+struct Reaction {
+  std::string equation;
+  int kind = kElementary;
+  bool reversible = true;
+  std::vector<std::pair<int, double>> reactants, products;  // (species, stoich)
+  double lnA = 0, b = 0, theta = 0, A = 0;                 // high-pressure / plain rate
+  double lnA0 = 0, b0 = 0, theta0 = 0, A0 = 0;             // low-pressure limit
+  std::vector<std::pair<int, double>> eff;                 // (species, efficiency - base)
+  double eff_base = 1.0;
+  double troe[4] = {0, 0, 0, 0};                           // a, T3, T1, T2
+  bool troe_has_t2 = false;
+};
+
+struct Mechanism {
+  std::string path;
+  std::vector<std::string> species;
+  std::vector<double> W, tmid;        // [K]
+  std::vector<double> nasa;           // [K][14]: low a0..a6, high a0..a6
+  std::vector<Reaction> reactions;
+  int K() const { return static_cast<int>(species.size()); }
+  int R() const { return static_cast<int>(reactions.size()); }
+};
+
+Mechanism load_mechanism_yaml(const std::string& path);
+
+// ---------------------------------------------------------------------------
+// Device tables. One contiguous blob; the kernel copies the "LDS part" into
+// shared memory once per workgroup and reads the "global part" (Jacobian
+// assembly lists) through the cache.
+// ---------------------------------------------------------------------------
+struct alignas(16) RxnHdr {     // 16 bytes
+  uint8_t sp[6];                // reactants 0..2, products 3..5 (0xFF = none)
+  uint8_t nu[6];                // integer stoichiometric coefficients
+  uint8_t kind;                 // RxnKind
+  uint8_t flags;                // bit0 reversible, bit1 constant kf (=A), bit2 Troe T2, bit3 constant k0
+  uint16_t pidx;                // index into pressure-dependent tables (pressure-dependent reactions)
+};
+
+struct alignas(8) PdepRec {     // per pressure-dependent reaction
+  double eff_base;
+  int32_t eff_begin, eff_end;   // range in eff tables
+  double lnA0, b0, theta0;      // low-pressure limit (falloff)
+  double troe_a, troe_T3, troe_T1, troe_T2;
+};
+
+// Layout of the blob, offsets in bytes from its start. Sizes in elements.
+struct DevMechLayout {
+  int32_t K, R, n_pdep, n_eff;
+  int32_t n_gather_lane;        // entries per lane in the balanced species gather
+  int32_t n_partials;           // partial-sum slots of the gather
+  int32_t n_slots;              // d(phi)/dC slots (Jacobian)
+  int32_t n_jterm_lane;         // Jacobian assembly terms per lane
+  int32_t n_coef;               // distinct stoichiometric coefficient values
+  int32_t lds_bytes;            // bytes [0, lds_bytes) are staged to LDS
+  // LDS part
+  int32_t off_species;          // double[K][16]: invW, W, tmid, (pad), nasa[14] -> see kSp*
+  int32_t off_hdr;              // RxnHdr[R]
+  int32_t off_arr;              // double[R][3]: lnA (or A), b, theta
+  int32_t off_pdep;             // PdepRec[n_pdep]
+  int32_t off_eff_val;          // double[n_eff]
+  int32_t off_eff_sp;           // uint8[n_eff] (padded to 8)
+  int32_t off_coef;             // double[16] distinct coefficient values
+  int32_t off_gather;           // uint32[n_gather_lane][64]  r | coef<<12 | flush<<16 | part<<17
+  int32_t off_part_ptr;         // uint16[K+1] partial-slot range per species
+  // global part
+  int32_t off_slot_base;        // uint16[R+1]: first d(phi)/dC slot of each reaction
+  int32_t off_jterm;            // uint64[n_jterm_lane][64]  cell | src<<16 | coef<<32 | last<<48
+  int32_t total_bytes;
+};
+
+constexpr int kSpStride = 17;   // doubles per species record: invW, W, tmid, nasa[14]
+
+struct DevMechBlob {
+  DevMechLayout lay{};
+  std::vector<uint8_t> bytes;
+};
+
+// Builds the packed tables. Throws std::runtime_error when the mechanism does not fit
+// the device path (more than 64 state variables, >3 species on a reaction side,
+// non-integer stoichiometry).
+DevMechBlob build_device_tables(const Mechanism& m);
+
+}  // namespace hpo

@@ -1,0 +1,119 @@
+/*
+ * hpc_ode_amd C ABI: batched stiff-chemistry BDF integration on one MI355X (gfx950).
+ *
+ * This is the drop-in boundary for the reference's per-cell CVODE path. The reference
+ * exposes that path through the C++ Integrator interface
+ * (include/ODEIntegrator/Integrators/Integrator.hpp:22-40: init / integrate(t0, t) / clean)
+ * with CVodeIntegrator (src/Integrators/CVodeIntegrator.cpp:34-155) as its serial
+ * implementation and CVodeIntegratorGPU (src/Integrators/CVodeIntegratorGPU.cpp:21-190)
+ * as its GPU one. The functions below are what such an Integrator calls: the C++
+ * class hpo::BDFIntegratorGPU (hpc_ode_amd/csrc/integrator.hpp) is the in-tree caller.
+ *
+ * Conventions: plain pointers and sizes; every function returns 0 on success or a
+ * negative HPO_E* code, with a message available from hpo_last_error() (thread-local).
+ * State rows are y = [T, Y_0 .. Y_{K-2}] (K species; the last species is recovered as
+ * 1 - sum Y, as at src/Integrators/CVodeIntegrator.cpp:90-96). Calls on one handle must
+ * not overlap in time (the caller serialises them); different handles are independent.
+ */
+#ifndef HPC_ODE_AMD_H
+#define HPC_ODE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPO_OK 0
+#define HPO_E_INVALID (-1)   /* bad argument */
+#define HPO_E_PARSE (-2)     /* mechanism file could not be read or is unsupported */
+#define HPO_E_DEVICE (-3)    /* HIP runtime error */
+#define HPO_E_NOMEM (-4)     /* device memory exhausted */
+#define HPO_E_UNSUPPORTED (-5)
+
+/* per-cell status codes: CVODE's return codes (cvode.h) */
+#define HPO_CELL_SUCCESS 0
+#define HPO_CELL_TOO_MUCH_WORK (-1)
+#define HPO_CELL_TOO_MUCH_ACC (-2)
+#define HPO_CELL_ERR_FAILURE (-3)
+#define HPO_CELL_CONV_FAILURE (-4)
+#define HPO_CELL_ILL_INPUT (-22)
+#define HPO_CELL_TOO_CLOSE (-27)
+
+#define HPO_NSTATS 8 /* per cell: nst, nfe, nje, nsetups, nni, ncfn, netf, q_last */
+
+typedef struct hpo_mech hpo_mech;
+typedef struct hpo_batch hpo_batch;
+
+typedef struct {
+  double rtol;             /* scalar relative tolerance (CVodeSStolerances) */
+  double atol;             /* scalar absolute tolerance */
+  int32_t maxsteps;        /* CVodeSetMaxNumSteps; reference uses 10000 */
+  int32_t lu_precision;    /* 0: FP64 Newton-matrix LU; 1: FP32 LU + FP64 residual refinement */
+  int32_t refine_sweeps;   /* refinement sweeps per linear solve when lu_precision == 1 */
+  int32_t waves_per_block; /* cells resident per workgroup (1..4); 0 = default */
+  int32_t blocks_per_cu;   /* persistent workgroups per CU; 0 = as many as fit */
+  int32_t sort_cells;      /* nonzero: work queue ordered by descending initial temperature */
+  int32_t reserved[2];
+} hpo_opts;
+
+typedef struct {
+  double kernel_ms;        /* integration kernel time, HIP events on the call's stream */
+  double total_ms;         /* whole call including queue reset and any host copies */
+  int64_t ncells;
+  int32_t grid, block;     /* persistent launch shape */
+  int32_t lds_bytes;       /* dynamic LDS per workgroup */
+  int32_t reserved;
+} hpo_timing;
+
+/* Defaults: rtol 1e-6, atol 1e-10, maxsteps 10000 (src/Integrators/CVodeIntegrator.cpp:107,122). */
+void hpo_opts_default(hpo_opts *o);
+
+/* ---- mechanism: replaces the compiled-in pyJac code (src/Mechanism/CPU, src/Mechanism/GPU) ---- */
+int hpo_mech_load_yaml(const char *path, hpo_mech **out);
+void hpo_mech_destroy(hpo_mech *m);
+/* n_state = n_species (T plus n_species - 1 mass fractions) */
+int hpo_mech_info(const hpo_mech *m, int32_t *n_species, int32_t *n_state, int32_t *n_reactions);
+int hpo_mech_species_name(const hpo_mech *m, int32_t k, char *buf, int32_t buflen);
+int hpo_mech_molecular_weights(const hpo_mech *m, double *W /* [n_species] */);
+/* Flat host tables (for cross-checking readers): see hpc_ode_amd/csrc/mechanism.hpp. */
+int hpo_mech_tables(const hpo_mech *m, double *nasa /*[K][14]*/, double *tmid /*[K]*/,
+                    double *arrhenius /*[R][8]: lnA,b,theta,A,lnA0,b0,theta0,A0*/,
+                    int32_t *kind /*[R]*/, int32_t *reversible /*[R]*/);
+
+/* ---- resident cell batch (device memory owned by the handle) ---- */
+int hpo_batch_create(const hpo_mech *m, int64_t capacity, hpo_batch **out);
+void hpo_batch_destroy(hpo_batch *b);
+/* Copies cells [first, first+count) from host. y: [count][ldy]; P and dt per cell, or NULL to
+ * use the scalar for every cell (the reference's single pressure / dt: Integrator.hpp:13-19). */
+int hpo_batch_put(hpo_batch *b, int64_t first, int64_t count, const double *y, int64_t ldy,
+                  const double *P, double P_scalar, const double *dt, double dt_scalar);
+/* Integrates cells [0, count) from t = 0 to their dt, in place on the device. */
+int hpo_batch_integrate(hpo_batch *b, int64_t count, const hpo_opts *o, void *hip_stream,
+                        hpo_timing *timing);
+int hpo_batch_get(const hpo_batch *b, int64_t first, int64_t count, double *y, int64_t ldy,
+                  int32_t *status /* [count] or NULL */, int32_t *stats /* [count][HPO_NSTATS] or NULL */);
+
+/* ---- device-pointer entry: all arrays already in device memory (e.g. torch tensors) ---- */
+int hpo_integrate_device(const hpo_mech *m, int64_t ncells, const double *d_y_in, double *d_y_out,
+                         int64_t ld, const double *d_P, const double *d_dt, int32_t *d_status,
+                         int32_t *d_stats, const int32_t *d_order, const hpo_opts *o, void *hip_stream,
+                         hpo_timing *timing);
+
+/* ---- component entry points (host arrays), used by parity tests ---- */
+/* f = dydt(y) for n states: y, f are [n][n_state]; the pyJac dydt of dydt.c:7-133. */
+int hpo_eval_rhs(const hpo_mech *m, int64_t n, const double *y, const double *P, double *f);
+/* J = d f / d y, [n][n_state][n_state] row-major; what eval_jacob (jacob.c:3-371) computes. */
+int hpo_eval_jac(const hpo_mech *m, int64_t n, const double *y, const double *P, double *J);
+/* Batched dense LU + solve with the kernel's register LU: M [n][ns][ns] row-major, b, x [n][ns]. */
+int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double *M, const double *b, double *x,
+                 int32_t *ok);
+
+const char *hpo_last_error(void);
+/* Device properties used for sizing: CU count, free/total HBM bytes. */
+int hpo_device_info(int32_t *n_cu, int64_t *free_bytes, int64_t *total_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
