@@ -14,8 +14,10 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhpc_ode.so")
+LIB_PATH = os.environ.get("HPO_LIB") or os.path.join(HERE, "libhpc_ode.so")
 NSTATS = 8
+NPROF = 5
+PROF_FIELDS = ("rhs", "jac", "lu", "solve", "cell")
 STATS_FIELDS = ("nst", "nfe", "nje", "nsetups", "nni", "ncfn", "netf", "qlast")
 
 CELL_STATUS = {0: "SUCCESS", -1: "TOO_MUCH_WORK", -2: "TOO_MUCH_ACC", -3: "ERR_FAILURE",
@@ -68,7 +70,7 @@ def lib():
                                     C.c_double]
         L.hpo_batch_integrate.argtypes = [vp, C.c_int64, C.POINTER(Opts), vp, C.POINTER(Timing)]
         L.hpo_batch_get.argtypes = [vp, C.c_int64, C.c_int64, _dp, C.c_int64, _ip, _ip]
-        L.hpo_integrate_device.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp,
+        L.hpo_integrate_device.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp, vp,
                                            C.POINTER(Opts), vp, C.POINTER(Timing)]
         L.hpo_eval_rhs.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
         L.hpo_eval_jac.argtypes = [vp, C.c_int64, _dp, _dp, _dp]

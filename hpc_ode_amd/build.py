@@ -21,7 +21,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "libhpc_ode.so")
+LIB = os.environ.get("HPO_LIB") or os.path.join(HERE, "libhpc_ode.so")
+EXTRA = os.environ.get("HPO_EXTRA_FLAGS", "").split()
 ARCH = os.environ.get("HPO_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -41,8 +42,8 @@ SOURCES = [
 
 
 def _compile(src, flags, verbose):
-    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *COMMON, *flags, "-c", os.path.join(CSRC, src), "-o", obj]
+    obj = os.path.join(BUILD, os.path.basename(src) + (".x" if EXTRA else "") + ".o")
+    cmd = [HIPCC, *COMMON, *flags, *EXTRA, "-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -69,17 +69,30 @@ HPO_DI void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-template <int N, typename T>
-HPO_DI T pick(const T (&v)[N], int i) {
-  T r = v[0];
-#pragma unroll
-  for (int j = 1; j < N; ++j) r = (i == j) ? v[j] : r;
-  return r;
+// Dynamic (wave-uniform) index into a small register array. Written as switches over
+// constant indices so that the array stays promotable to registers: a loop of selects is
+// folded back into an indexed load, which forces the array into scratch memory.
+template <typename T>
+HPO_DI T pick(const T (&v)[6], int i) {
+  switch (i) {
+    case 0: return v[0];
+    case 1: return v[1];
+    case 2: return v[2];
+    case 3: return v[3];
+    case 4: return v[4];
+    default: return v[5];
+  }
 }
-template <int N, typename T>
-HPO_DI void put(T (&v)[N], int i, T x) {
-#pragma unroll
-  for (int j = 0; j < N; ++j) v[j] = (i == j) ? x : v[j];
+template <typename T>
+HPO_DI void put(T (&v)[6], int i, T x) {
+  switch (i) {
+    case 0: v[0] = x; break;
+    case 1: v[1] = x; break;
+    case 2: v[2] = x; break;
+    case 3: v[3] = x; break;
+    case 4: v[4] = x; break;
+    default: v[5] = x; break;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -568,10 +581,14 @@ struct LuReg {
   int perm;   // lane k: pivot lane of step k
 };
 
+// Branch-free over the elimination steps: a zero or NaN pivot column marks the matrix
+// singular and turns the remaining updates into no-ops instead of leaving the loop, so the
+// unrolled elimination has a single exit.
 template <int NS, typename F>
 HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
   int pos = lane;
   bool done = false;
+  bool singular = false;
   L.step = 64;
   L.perm = 0;
   L.rdiag = F(0);
@@ -582,17 +599,19 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
     F kmax = wmax(key);
     int ppos = key_pos(kmax);
     unsigned long long bal = __ballot(cand && pos == ppos);
-    if (bal == 0ull) return false;  // NaN column
-    int p = __ffsll(bal) - 1;
+    int p = bal ? (__ffsll(bal) - 1) : 0;
     F piv = rl(L.a[k], p);
-    if (piv == F(0)) return false;
-    F mult = F(1) / piv;
-    if (pos == k) pos = ppos;
-    if (lane == p) {
-      pos = k;
-      done = true;
-      L.step = k;
-      L.rdiag = mult;
+    bool bad = (bal == 0ull) || (piv == F(0));
+    singular = singular || bad;
+    F mult = bad ? F(0) : F(1) / piv;
+    if (!bad) {
+      if (pos == k) pos = ppos;
+      if (lane == p) {
+        pos = k;
+        done = true;
+        L.step = k;
+        L.rdiag = mult;
+      }
     }
     if (lane == k) L.perm = p;
     F l = done ? F(0) : L.a[k] * mult;
@@ -601,9 +620,13 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
     for (int j = k + 1; j < NS; ++j) {
       F pj = rl(L.a[j], p);
       L.a[j] = fma(-l, pj, L.a[j]);
+      // The pivot-row broadcasts land in SGPRs; scheduling them in groups of 8 keeps
+      // them from being hoisted en masse past the 102-entry SGPR file.
+      if ((j - k) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
-  return true;
+  return !singular;
 }
 
 template <int NS, typename F>
@@ -614,6 +637,7 @@ HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane) {
     F yk = rl(b, p);
     F c = (L.step > k) ? L.a[k] : F(0);
     b = fma(-c, yk, b);
+    __builtin_amdgcn_sched_barrier(0);
   }
   F x = F(0);
 #pragma unroll
@@ -623,6 +647,7 @@ HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane) {
     x = (lane == k) ? xk : x;
     F c = (L.step < k) ? L.a[k] : F(0);
     b = fma(-c, xk, b);
+    __builtin_amdgcn_sched_barrier(0);
   }
   return x;
 }

@@ -13,6 +13,13 @@
 #include "bdf_device.hpp"
 #include "kernels.hpp"
 
+// Occupancy target of the BDF kernel (waves per SIMD). At 1 the kernel may use the whole
+// 512-entry unified VGPR/AGPR file and keeps the Newton matrix row, the Nordsieck history
+// and the RHS working set in registers without scratch spills.
+#ifndef HPO_WAVES_PER_EU
+#define HPO_WAVES_PER_EU 1
+#endif
+
 namespace hpo {
 namespace dev {
 
@@ -100,6 +107,8 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
   int ncf = 0, nef = 0, nflag = FIRST_CALL, m = 0, need_setup = 0, jbad = 0;
   int status = HPO_SUCCESS;
   int count1 = 0;
+  const bool prof = A.prof != nullptr;
+  long long c_rhs = 0, c_jac = 0, c_lu = 0, c_sol = 0, c_t0 = prof ? clock64() : 0, c_a = 0;
 
   // error weights for y0 (cvInitialSetup)
   {
@@ -422,7 +431,9 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
   int action = (status == HPO_SUCCESS) ? ACT_EVAL : ACT_DONE;
   while (action != ACT_DONE) {
     // ---- the single RHS site ----
+    if (prof) c_a = clock64();
     double f = rhs(t, w, yv, P, lane, n);
+    if (prof) c_rhs += clock64() - c_a;
     nfe++;
     action = ACT_DONE;
     if (phase == PH_NEWTON) {
@@ -438,10 +449,13 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
                     (convfail == FAIL_OTHER);
         if constexpr (LU32) {
           if (jnew) {
+            if (prof) c_a = clock64();
             jacobian<NS>(t, w, g, yv, P, lane);
+            if (prof) c_jac += clock64() - c_a;
             nje++;
             nstlj = nst;
           }
+          if (prof) c_a = clock64();
 #pragma unroll
           for (int j = 0; j < NS; ++j) {
             double Jv = g.J[j * 64 + lane];
@@ -450,15 +464,19 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           }
         } else {
           if (jnew) {
+            if (prof) c_a = clock64();
             jacobian<NS>(t, w, g, yv, P, lane);
+            if (prof) c_jac += clock64() - c_a;
             nje++;
             nstlj = nst;
           }
+          if (prof) c_a = clock64();
 #pragma unroll
           for (int j = 0; j < NS; ++j) lu.L.a[j] = -gamma * g.J[j * 64 + lane] + ((j == lane) ? 1.0 : 0.0);
         }
         jcur = jnew ? 1 : 0;
         bool ok = lu_factor<NS>(lu.L, lane);
+        if (prof) c_lu += clock64() - c_a;
         S.gamma_lu = gamma;
         nsetups++;
         S.gamrat = 1.0;
@@ -473,6 +491,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         nni++;
         double b = -delta;
         double x;
+        if (prof) c_a = clock64();
         if constexpr (LU32) {
           float x32 = lu_solve<NS>(lu.L, static_cast<float>(b), lane);
           x = static_cast<double>(x32);
@@ -489,6 +508,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         } else {
           x = lu_solve<NS>(lu.L, b, lane);
         }
+        if (prof) c_sol += clock64() - c_a;
         if (!act) x = 0.0;
         const double gamrat = S.gamrat;
         if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
@@ -628,12 +648,20 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
       st[6] = netf;
       st[7] = q;
     }
+    if (prof) {
+      int64_t* pr = A.prof + cell * HPO_NPROF;
+      pr[0] = c_rhs;
+      pr[1] = c_jac;
+      pr[2] = c_lu;
+      pr[3] = c_sol;
+      pr[4] = clock64() - c_t0;
+    }
   }
   wave_sync();
 }
 
 template <int NS, bool LU32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void bdf_kernel(BdfArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPO_WAVES_PER_EU, HPO_WAVES_PER_EU))) void bdf_kernel(BdfArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
   const DevMechLayout& Lay = A.lay;
   for (int i = threadIdx.x * 16; i < Lay.lds_bytes; i += blockDim.x * 16)

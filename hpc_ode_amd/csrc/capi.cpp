@@ -136,7 +136,8 @@ void check_opts(const hpo_opts& o) {
 // Launches the persistent BDF kernel on arrays already resident on the device.
 void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* d_yout, int64_t ld,
                       const double* d_P, const double* d_dt, int32_t* d_status, int32_t* d_stats,
-                      const int32_t* d_order, const hpo_opts& o, hipStream_t st, hpo_timing* tm) {
+                      const int32_t* d_order, int64_t* d_prof, const hpo_opts& o, hipStream_t st,
+                      hpo_timing* tm) {
   check_opts(o);
   const int ns = m->blob.lay.K;
   if (!hpo::kernel_supports(ns))
@@ -177,6 +178,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   A.atol = o.atol;
   A.maxsteps = o.maxsteps;
   A.refine_sweeps = o.refine_sweeps;
+  A.prof = reinterpret_cast<int64_t*>(d_prof);
 
   HPO_HIP(hipMemsetAsync(m->d_queue, 0, sizeof(uint32_t), st));
   HPO_HIP(hipEventRecord(m->ev0, st));
@@ -348,8 +350,8 @@ int hpo_batch_integrate(hpo_batch* b, int64_t count, const hpo_opts* o, void* st
       order = b->d_order;
     }
     hpo_timing local{};
-    launch_integrate(m, count, b->d_y, b->d_y, b->ld, b->d_P, b->d_dt, b->d_status, b->d_stats, order, opt, st,
-                     &local);
+    launch_integrate(m, count, b->d_y, b->d_y, b->ld, b->d_P, b->d_dt, b->d_status, b->d_stats, order, nullptr,
+                     opt, st, &local);
     if (tm) *tm = local;
     return HPO_OK;
   } catch (...) {
@@ -382,13 +384,14 @@ int hpo_batch_get(const hpo_batch* b, int64_t first, int64_t count, double* y, i
 
 int hpo_integrate_device(const hpo_mech* mc, int64_t ncells, const double* d_y_in, double* d_y_out, int64_t ld,
                          const double* d_P, const double* d_dt, int32_t* d_status, int32_t* d_stats,
-                         const int32_t* d_order, const hpo_opts* o, void* stream, hpo_timing* tm) {
+                         const int32_t* d_order, int64_t* d_prof, const hpo_opts* o, void* stream,
+                         hpo_timing* tm) {
   if (!mc || !d_y_in || !d_y_out || !d_P || !d_dt || !d_status) return fail(HPO_E_INVALID, "null argument");
   hpo_opts def;
   hpo_opts_default(&def);
   try {
     launch_integrate(const_cast<hpo_mech*>(mc), ncells, d_y_in, d_y_out, ld, d_P, d_dt, d_status, d_stats, d_order,
-                     o ? *o : def, static_cast<hipStream_t>(stream), tm);
+                     d_prof, o ? *o : def, static_cast<hipStream_t>(stream), tm);
     return HPO_OK;
   } catch (...) {
     return map_exceptions();

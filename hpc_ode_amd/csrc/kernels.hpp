@@ -16,6 +16,7 @@
 #define HPO_TOO_CLOSE (-27)
 
 #define HPO_NSTATS 8  // nst, nfe, nje, nsetups, nni, ncfn, netf, q_last
+#define HPO_NPROF 5   // cycles in: RHS, Jacobian, LU factorization, LU solves, whole cell
 
 namespace hpo {
 
@@ -37,6 +38,7 @@ struct BdfArgs {
   double rtol, atol;
   int32_t maxsteps;
   int32_t refine_sweeps;
+  int64_t* prof;             // [ncells][HPO_NPROF] shader-clock cycles per phase, or null
 };
 
 struct EvalArgs {

@@ -95,10 +95,15 @@ int hpo_batch_get(const hpo_batch *b, int64_t first, int64_t count, double *y, i
                   int32_t *status /* [count] or NULL */, int32_t *stats /* [count][HPO_NSTATS] or NULL */);
 
 /* ---- device-pointer entry: all arrays already in device memory (e.g. torch tensors) ---- */
+/* d_order: work-queue order of the cells (a permutation of 0..ncells-1) or NULL.
+ * d_prof: [ncells][HPO_NPROF] shader-clock cycles per phase (RHS, Jacobian, LU factorization,
+ * LU solves, whole cell) or NULL. With timing != NULL the call waits for the kernel;
+ * otherwise it returns after the launch. */
+#define HPO_NPROF 5
 int hpo_integrate_device(const hpo_mech *m, int64_t ncells, const double *d_y_in, double *d_y_out,
                          int64_t ld, const double *d_P, const double *d_dt, int32_t *d_status,
-                         int32_t *d_stats, const int32_t *d_order, const hpo_opts *o, void *hip_stream,
-                         hpo_timing *timing);
+                         int32_t *d_stats, const int32_t *d_order, int64_t *d_prof, const hpo_opts *o,
+                         void *hip_stream, hpo_timing *timing);
 
 /* ---- component entry points (host arrays), used by parity tests ---- */
 /* f = dydt(y) for n states: y, f are [n][n_state]; the pyJac dydt of dydt.c:7-133. */

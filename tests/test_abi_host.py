@@ -1,0 +1,66 @@
+"""Host-side checks of the C ABI (no GPU calls): the library loads, exports every function
+include/hpc_ode.h declares, parses mechanisms like the oracle does, and reports errors."""
+import ctypes as C
+import subprocess
+
+import numpy as np
+import pytest
+
+import hpc_ode_amd as H
+import oracle as O
+
+
+def test_library_exports_every_declared_symbol():
+    syms = H.exported_symbols()
+    assert len(syms) >= 18
+    out = subprocess.run(["nm", "-D", "--defined-only", H.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+
+
+def test_no_oracle_in_product_library():
+    out = subprocess.run(["nm", "-D", H.LIB_PATH], capture_output=True, text=True).stdout
+    assert "or_cvode" not in out and "or_rhs" not in out
+    ldd = subprocess.run(["ldd", H.LIB_PATH], capture_output=True, text=True).stdout
+    assert "liboracle" not in ldd and "pyjac" not in ldd
+
+
+@pytest.mark.parametrize("which", ["gri", "h2"])
+def test_cpp_reader_matches_oracle_reader(which, gri_path, h2_path):
+    """The product's C++ YAML reader and the oracle's PyYAML reader give identical tables."""
+    path = gri_path if which == "gri" else h2_path
+    m = H.Mechanism(path)
+    o = O.Oracle(path)
+    assert m.species == o.mech.species
+    assert np.array_equal(m.W, o.t.W)
+    t = m.tables()
+    assert np.array_equal(t["tmid"], o.t.tmid)
+    assert np.array_equal(t["nasa"], o.t.nasa)
+    assert np.array_equal(t["kind"], o.t.kind)
+    assert np.array_equal(t["rev"], o.t.rev)
+    fall = o.t.kind >= 2
+    # lnA, b, theta, A of every reaction; low-pressure parameters of falloff reactions
+    assert np.allclose(t["arr"][:, :4], o.t.arr[:, :4], rtol=1e-15, atol=0)
+    assert np.allclose(t["arr"][fall, 4:], o.t.arr[fall, 4:], rtol=1e-15, atol=0)
+
+
+def test_reader_errors_are_reported(tmp_path):
+    with pytest.raises(H.HpoError, match="cannot open"):
+        H.Mechanism(str(tmp_path / "missing.yaml"))
+    bad = tmp_path / "bad.yaml"
+    bad.write_text("phases:\n- name: x\n  species: [A]\nspecies:\n- name: A\n  composition: {Zz: 1}\n"
+                   "  thermo: {model: NASA7, temperature-ranges: [200, 1000, 3000], data: [[1,2,3,4,5,6,7]]}\n"
+                   "reactions: []\n")
+    with pytest.raises(H.HpoError, match="unknown element"):
+        H.Mechanism(str(bad))
+
+
+def test_invalid_arguments_return_error_codes():
+    L = H.lib()
+    assert L.hpo_mech_load_yaml(None, None) == -1
+    assert L.hpo_batch_create(None, 10, None) == -1
+    o = H.Opts()
+    L.hpo_opts_default(C.byref(o))
+    assert o.rtol == 1e-6 and o.atol == 1e-10 and o.maxsteps == 10000
+    assert L.hpo_lu_solve(7, 0, 1, None, None, None, None) == -1
