@@ -63,6 +63,15 @@ HPO_DI F wmax(F v) {
 
 HPO_DI double shfl(double v, int src) { return __shfl(v, src, 64); }
 
+// Returns v unchanged but hides its value from the optimiser. Used on the lane index where
+// lane-dependent constants (identity-matrix entries, per-lane addresses) are formed, so
+// loop-invariant code motion does not hoist dozens of them out of the cell loop and keep
+// them live (and spilled) for the whole kernel.
+HPO_DI int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // Orders LDS and scratch accesses of the lanes of one wave (no workgroup barrier).
 HPO_DI void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -138,7 +147,7 @@ HPO_DI Tabs make_tabs(const uint8_t* lds, const uint8_t* glob, const DevMechLayo
 struct WaveLds {
   double *C, *g, *hRT, *cp, *phi, *part;
 };
-__host__ __device__ inline int wave_lds_doubles(int R, int npart) { return 256 + ((R + 1) & ~1) + 3 * ((npart + 1) & ~1); }
+__host__ __device__ inline int wave_lds_doubles(int R, int npart) { return 256 + ((R + 1) & ~1) + 3 * ((npart + 2) & ~1); }
 HPO_DI WaveLds make_wave_lds(double* base, int R, int npart) {
   WaveLds w;
   w.C = base;
@@ -212,11 +221,12 @@ HPO_DI void species_state(const Tabs& t, double y, double P, int lane, SpState& 
   s.dcpR = a[1] + T * (2.0 * a[2] + T * (3.0 * a[3] + T * 4.0 * a[4]));
 }
 
+// c^nu for nu in 0..3 without branches (nu = 0 marks an unused reaction slot)
 HPO_DI double cpow_int(double c, int nu) {
-  double p = c;
-  if (nu >= 2) p = c * c;
-  if (nu >= 3) p *= c;
-  return p;
+  double c2 = c * c;
+  double p = (nu >= 2) ? c2 : c;
+  p = (nu >= 3) ? p * c : p;
+  return (nu == 0) ? 1.0 : p;
 }
 HPO_DI double dcpow_int(double c, int nu) { return nu == 1 ? 1.0 : (nu == 2 ? 2.0 * c : 3.0 * c * c); }
 
@@ -232,20 +242,19 @@ HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part
   double acc[NSRC];
 #pragma unroll
   for (int s = 0; s < NSRC; ++s) acc[s] = 0.0;
-#pragma unroll 1
+#pragma unroll 4
   for (int i = 0; i < t.ngl; ++i) {
     uint32_t e = t.gather[i * 64 + lane];
     double c = t.coef[(e >> 12) & 15u];
     int r = static_cast<int>(e & 4095u);
+    // flush the partial sum at the end of a species run; other entries write a scratch slot
+    const bool flush = (e >> 16) & 1u;
+    const int p = flush ? static_cast<int>(e >> 17) : t.npart;
 #pragma unroll
-    for (int s = 0; s < NSRC; ++s) acc[s] += c * src[s][r];
-    if (e & (1u << 16)) {
-      int p = static_cast<int>(e >> 17);
-#pragma unroll
-      for (int s = 0; s < NSRC; ++s) {
-        part[s * t.npart + p] = acc[s];
-        acc[s] = 0.0;
-      }
+    for (int s = 0; s < NSRC; ++s) {
+      acc[s] += c * src[s][r];
+      part[s * (t.npart + 1) + p] = acc[s];
+      acc[s] = flush ? 0.0 : acc[s];
     }
   }
   wave_sync();
@@ -255,10 +264,58 @@ HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part
     int b = t.pptr[lane], e = t.pptr[lane + 1];
     for (int p = b; p < e; ++p) {
 #pragma unroll
-      for (int s = 0; s < NSRC; ++s) out[s] += part[s * t.npart + p];
+      for (int s = 0; s < NSRC; ++s) out[s] += part[s * (t.npart + 1) + p];
     }
   }
   wave_sync();
+}
+
+// Net rate (fwd - rev) * pressure modification of reaction r (rxn_rates.c, rxn_rates_pres_mod.c).
+HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, int r) {
+  const RxnHdr h = t.hdr[r];
+  const double* ar = t.arr + 3 * r;
+  // straight-line for the elementary part, so two reactions per lane can be interleaved
+  double e = exp(ar[0] + ar[1] * s.lnT - ar[2] * s.invT);
+  double kf = (h.flags & 2) ? ar[0] : e;
+  double fwd = kf, pc = 1.0, dG = 0.0, dnu = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    fwd *= cpow_int(w.C[h.sp[q]], h.nu[q]);
+    dG -= h.nu[q] * w.g[h.sp[q]];
+    dnu -= h.nu[q];
+  }
+#pragma unroll
+  for (int q = 3; q < 6; ++q) {
+    pc *= cpow_int(w.C[h.sp[q]], h.nu[q]);
+    dG += h.nu[q] * w.g[h.sp[q]];
+    dnu += h.nu[q];
+  }
+  // k_r = k_f / Kc, Kc = (P_atm/(R T))^dnu exp(-dG)
+  double rev = (h.flags & 1) ? kf * exp(dG - dnu * s.lnPfac) * pc : 0.0;
+  double pm = 1.0;
+  if (h.kind != kElementary) {
+    const PdepRec& pr = t.pd[h.pidx];
+    double M = pr.eff_base * s.m;
+    for (int e2 = pr.eff_begin; e2 < pr.eff_end; ++e2) M += t.effv[e2] * w.C[t.effs[e2]];
+    if (h.kind == kThreeBody) {
+      pm = M;
+    } else {
+      double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, h.flags & 8, s.lnT, s.invT);
+      double Pr = k0 * M / kf;
+      double F = 1.0;
+      if (h.kind == kTroe) {
+        double Fc = (1.0 - pr.troe_a) * exp(-s.T / pr.troe_T3) + pr.troe_a * exp(-s.T / pr.troe_T1) +
+                    ((h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0);
+        double logFc = log10(fmax(Fc, 1.0e-300));
+        double x = log10(fmax(Pr, 1.0e-300));
+        double A = x - 0.67 * logFc - 0.4;
+        double B = 0.806 - 1.1762 * logFc - 0.14 * x;
+        F = exp10(logFc / (1.0 + A * A / (B * B)));
+      }
+      pm = F * Pr / (1.0 + Pr);
+    }
+  }
+  return (fwd - rev) * pm;
 }
 
 // ---------------------------------------------------------------------------
@@ -272,59 +329,19 @@ HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane,
     w.C[lane] = s.rho * s.Yk * s.invW;
     w.g[lane] = s.hRT - s.sR;
   }
+  if (lane == kDummySpecies) {
+    w.C[lane] = 1.0;
+    w.g[lane] = 0.0;
+  }
   wave_sync();
+  // two reactions per lane per pass: independent exp/log chains the scheduler can overlap
 #pragma unroll 1
-  for (int r = lane; r < R; r += 64) {
-    const RxnHdr h = t.hdr[r];
-    const double* ar = t.arr + 3 * r;
-    double kf = arrh(ar[0], ar[1], ar[2], h.flags & 2, s.lnT, s.invT);
-    double fwd = kf;
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-      if (h.sp[q] != 0xFF) fwd *= cpow_int(w.C[h.sp[q]], h.nu[q]);
-    double rev = 0.0;
-    if (h.flags & 1) {
-      double dG = 0.0, dnu = 0.0, pc = 1.0;
-#pragma unroll
-      for (int q = 3; q < 6; ++q)
-        if (h.sp[q] != 0xFF) {
-          dG += h.nu[q] * w.g[h.sp[q]];
-          dnu += h.nu[q];
-          pc *= cpow_int(w.C[h.sp[q]], h.nu[q]);
-        }
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-        if (h.sp[q] != 0xFF) {
-          dG -= h.nu[q] * w.g[h.sp[q]];
-          dnu -= h.nu[q];
-        }
-      // k_r = k_f / Kc, Kc = (P_atm/(R T))^dnu exp(-dG)
-      rev = kf * exp(dG - dnu * s.lnPfac) * pc;
-    }
-    double pm = 1.0;
-    if (h.kind != kElementary) {
-      const PdepRec& pr = t.pd[h.pidx];
-      double M = pr.eff_base * s.m;
-      for (int e = pr.eff_begin; e < pr.eff_end; ++e) M += t.effv[e] * w.C[t.effs[e]];
-      if (h.kind == kThreeBody) {
-        pm = M;
-      } else {
-        double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, h.flags & 8, s.lnT, s.invT);
-        double Pr = k0 * M / kf;
-        double F = 1.0;
-        if (h.kind == kTroe) {
-          double Fc = (1.0 - pr.troe_a) * exp(-s.T / pr.troe_T3) + pr.troe_a * exp(-s.T / pr.troe_T1) +
-                      ((h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0);
-          double logFc = log10(fmax(Fc, 1.0e-300));
-          double x = log10(fmax(Pr, 1.0e-300));
-          double A = x - 0.67 * logFc - 0.4;
-          double B = 0.806 - 1.1762 * logFc - 0.14 * x;
-          F = exp10(logFc / (1.0 + A * A / (B * B)));
-        }
-        pm = F * Pr / (1.0 + Pr);
-      }
-    }
-    w.phi[r] = (fwd - rev) * pm;
+  for (int r = lane; r < R; r += 128) {
+    const int r2 = r + 64;
+    double p1 = reaction_rate(t, w, s, r);
+    double p2 = (r2 < R) ? reaction_rate(t, w, s, r2) : 0.0;
+    w.phi[r] = p1;
+    if (r2 < R) w.phi[r2] = p2;
   }
   wave_sync();
   const double* src[1] = {w.phi};
@@ -357,6 +374,11 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
     w.hRT[lane] = s.hRT;
     w.cp[lane] = cpk;
   }
+  if (lane == kDummySpecies) {
+    w.C[lane] = 1.0;
+    w.g[lane] = 0.0;
+    w.hRT[lane] = 0.0;
+  }
   // zero the dense d wdot/dC buffer (columns 0..K-1, rows 0..63)
   for (int i = 0; i < K; ++i) g.JC[i * 64 + lane] = 0.0;
   wave_sync();
@@ -372,16 +394,16 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
     double fwd = kf;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      cr[q] = (h.sp[q] != 0xFF) ? w.C[h.sp[q]] : 1.0;
-      if (h.sp[q] != 0xFF) fwd *= cpow_int(cr[q], h.nu[q]);
+      cr[q] = w.C[h.sp[q]];
+      fwd *= cpow_int(cr[q], h.nu[q]);
     }
     double kr = 0.0, rev = 0.0, dlnkr = 0.0, dH = 0.0;
     {
       double dG = 0.0, dnu = 0.0, dHRT = 0.0;
 #pragma unroll
       for (int q = 3; q < 6; ++q) {
-        cp_[q - 3] = (h.sp[q] != 0xFF) ? w.C[h.sp[q]] : 1.0;
-        if (h.sp[q] != 0xFF) {
+        cp_[q - 3] = w.C[h.sp[q]];
+        if (h.nu[q]) {
           dG += h.nu[q] * w.g[h.sp[q]];
           dHRT += h.nu[q] * w.hRT[h.sp[q]];
           dnu += h.nu[q];
@@ -389,7 +411,7 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
       }
 #pragma unroll
       for (int q = 0; q < 3; ++q)
-        if (h.sp[q] != 0xFF) {
+        if (h.nu[q]) {
           dG -= h.nu[q] * w.g[h.sp[q]];
           dHRT -= h.nu[q] * w.hRT[h.sp[q]];
           dnu -= h.nu[q];
@@ -400,7 +422,7 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
         rev = kr;
 #pragma unroll
         for (int q = 0; q < 3; ++q)
-          if (h.sp[3 + q] != 0xFF) rev *= cpow_int(cp_[q], h.nu[3 + q]);
+          rev *= cpow_int(cp_[q], h.nu[3 + q]);
         dlnkr = dlnkf - (dHRT - dnu) * s.invT;
       }
     }
@@ -458,21 +480,21 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
     // reactant slots: pm * d fwd / dC
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      if (h.sp[a] != 0xFF) {
+      if (h.nu[a]) {
         double d = kf;
 #pragma unroll
         for (int b = 0; b < 3; ++b)
-          if (h.sp[b] != 0xFF) d *= (b == a) ? dcpow_int(cr[b], h.nu[b]) : cpow_int(cr[b], h.nu[b]);
+          if (h.nu[b]) d *= (b == a) ? dcpow_int(cr[b], h.nu[b]) : cpow_int(cr[b], h.nu[b]);
         g.dphi[sb++] = pm * d;
       }
     if (h.flags & 1) {
 #pragma unroll
       for (int a = 0; a < 3; ++a)
-        if (h.sp[3 + a] != 0xFF) {
+        if (h.nu[3 + a]) {
           double d = kr;
 #pragma unroll
           for (int b = 0; b < 3; ++b)
-            if (h.sp[3 + b] != 0xFF) d *= (b == a) ? dcpow_int(cp_[b], h.nu[3 + b]) : cpow_int(cp_[b], h.nu[3 + b]);
+            if (h.nu[3 + b]) d *= (b == a) ? dcpow_int(cp_[b], h.nu[3 + b]) : cpow_int(cp_[b], h.nu[3 + b]);
           g.dphi[sb++] = -pm * d;
         }
     }
@@ -620,11 +642,7 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
     for (int j = k + 1; j < NS; ++j) {
       F pj = rl(L.a[j], p);
       L.a[j] = fma(-l, pj, L.a[j]);
-      // The pivot-row broadcasts land in SGPRs; scheduling them in groups of 8 keeps
-      // them from being hoisted en masse past the 102-entry SGPR file.
-      if ((j - k) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
     }
-    __builtin_amdgcn_sched_barrier(0);
   }
   return !singular;
 }
@@ -637,7 +655,6 @@ HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane) {
     F yk = rl(b, p);
     F c = (L.step > k) ? L.a[k] : F(0);
     b = fma(-c, yk, b);
-    __builtin_amdgcn_sched_barrier(0);
   }
   F x = F(0);
 #pragma unroll
@@ -647,7 +664,6 @@ HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane) {
     x = (lane == k) ? xk : x;
     F c = (L.step < k) ? L.a[k] : F(0);
     b = fma(-c, xk, b);
-    __builtin_amdgcn_sched_barrier(0);
   }
   return x;
 }

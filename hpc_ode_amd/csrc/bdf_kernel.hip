@@ -13,11 +13,16 @@
 #include "bdf_device.hpp"
 #include "kernels.hpp"
 
-// Occupancy target of the BDF kernel (waves per SIMD). At 1 the kernel may use the whole
-// 512-entry unified VGPR/AGPR file and keeps the Newton matrix row, the Nordsieck history
-// and the RHS working set in registers without scratch spills.
-#ifndef HPO_WAVES_PER_EU
-#define HPO_WAVES_PER_EU 1
+// Occupancy targets of the BDF kernel (waves per SIMD). At 1 a wave may use the whole
+// 512-entry unified VGPR/AGPR file: the FP64 path keeps its 106-register Newton-matrix row,
+// the Nordsieck history and the RHS working set there without scratch spills. The FP32-LU
+// path holds a 53-register row; at 2 waves per SIMD it still spills (the RHS and solve
+// working sets peak together), so both default to 1.
+#ifndef HPO_WAVES_PER_EU_FP64
+#define HPO_WAVES_PER_EU_FP64 1
+#endif
+#ifndef HPO_WAVES_PER_EU_FP32
+#define HPO_WAVES_PER_EU_FP32 1
 #endif
 
 namespace hpo {
@@ -49,7 +54,6 @@ struct LuStore<NS, false> {
 template <int NS>
 struct LuStore<NS, true> {
   LuReg<NS, float> L;
-  float J[NS];  // Jacobian row kept for the FP64-arithmetic refinement residual
 };
 
 HPO_DI double wrms(double v, double w, int n) {
@@ -444,6 +448,8 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         if (jbad) convfail = FAIL_BAD_J;
         // cvLsSetup / cvLsLinSys
         const double gamma = S.gamma;
+        const int ol = opaque(lane);
+        const double* Jrow = g.J + ol;
         double dgamma = fabs(gamma / S.gammap - 1.0);
         bool jnew = (nst == 0) || (nst >= nstlj + MSBJ) || (convfail == FAIL_BAD_J && dgamma < DGMAX_JBAD) ||
                     (convfail == FAIL_OTHER);
@@ -458,9 +464,8 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           if (prof) c_a = clock64();
 #pragma unroll
           for (int j = 0; j < NS; ++j) {
-            double Jv = g.J[j * 64 + lane];
-            lu.J[j] = static_cast<float>(Jv);
-            lu.L.a[j] = static_cast<float>(-gamma * Jv + ((j == lane) ? 1.0 : 0.0));
+            double Jv = Jrow[j * 64];
+            lu.L.a[j] = static_cast<float>(-gamma * Jv + ((j == ol) ? 1.0 : 0.0));
           }
         } else {
           if (jnew) {
@@ -472,7 +477,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           }
           if (prof) c_a = clock64();
 #pragma unroll
-          for (int j = 0; j < NS; ++j) lu.L.a[j] = -gamma * g.J[j * 64 + lane] + ((j == lane) ? 1.0 : 0.0);
+          for (int j = 0; j < NS; ++j) lu.L.a[j] = -gamma * Jrow[j * 64] + ((j == ol) ? 1.0 : 0.0);
         }
         jcur = jnew ? 1 : 0;
         bool ok = lu_factor<NS>(lu.L, lane);
@@ -496,11 +501,13 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           float x32 = lu_solve<NS>(lu.L, static_cast<float>(b), lane);
           x = static_cast<double>(x32);
           const double gl = S.gamma_lu;
+          const double* Jrow = g.J + opaque(lane);
           for (int sw = 0; sw < A.refine_sweeps; ++sw) {
-            // r = b - (I - gamma J) x in FP64 arithmetic
+            // r = b - (I - gamma J) x in FP64, with the FP64 Jacobian row kept in this
+            // wave's scratch (cache-resident) rather than in registers
             double Jx = 0.0;
-#pragma unroll
-            for (int j = 0; j < NS; ++j) Jx = fma(static_cast<double>(lu.J[j]), rl(x, j), Jx);
+#pragma unroll 8
+            for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j * 64], rl(x, j), Jx);
             double r = b - x + gl * Jx;
             float d32 = lu_solve<NS>(lu.L, static_cast<float>(r), lane);
             x += static_cast<double>(d32);
@@ -661,7 +668,9 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
 }
 
 template <int NS, bool LU32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPO_WAVES_PER_EU, HPO_WAVES_PER_EU))) void bdf_kernel(BdfArgs A) {
+__global__ __launch_bounds__(256) __attribute__((
+    amdgpu_waves_per_eu(LU32 ? HPO_WAVES_PER_EU_FP32 : HPO_WAVES_PER_EU_FP64,
+                        LU32 ? HPO_WAVES_PER_EU_FP32 : HPO_WAVES_PER_EU_FP64))) void bdf_kernel(BdfArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
   const DevMechLayout& Lay = A.lay;
   for (int i = threadIdx.x * 16; i < Lay.lds_bytes; i += blockDim.x * 16)

@@ -470,7 +470,7 @@ DevMechBlob build_device_tables(const Mechanism& m) {
     const Reaction& rx = *rx_of[r];
     RxnHdr h;
     std::memset(&h, 0, sizeof(h));
-    for (int s = 0; s < 6; ++s) h.sp[s] = 0xFF;
+    for (int s = 0; s < 6; ++s) h.sp[s] = kDummySpecies;  // nu = 0: contributes C^0 = 1, 0*g
     for (size_t s = 0; s < rx.reactants.size(); ++s) {
       h.sp[s] = static_cast<uint8_t>(rx.reactants[s].first);
       h.nu[s] = static_cast<uint8_t>(rx.reactants[s].second);

@@ -53,8 +53,10 @@ Mechanism load_mechanism_yaml(const std::string& path);
 // shared memory once per workgroup and reads the "global part" (Jacobian
 // assembly lists) through the cache.
 // ---------------------------------------------------------------------------
+constexpr uint8_t kDummySpecies = 63;  // unused reaction slots point here (concentration 1, g 0)
+
 struct alignas(16) RxnHdr {     // 16 bytes
-  uint8_t sp[6];                // reactants 0..2, products 3..5 (0xFF = none)
+  uint8_t sp[6];                // reactants 0..2, products 3..5 (unused: kDummySpecies with nu = 0)
   uint8_t nu[6];                // integer stoichiometric coefficients
   uint8_t kind;                 // RxnKind
   uint8_t flags;                // bit0 reversible, bit1 constant kf (=A), bit2 Troe T2, bit3 constant k0
