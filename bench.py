@@ -42,7 +42,24 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="per-phase cycle counters (stderr)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def measured_traffic(ncells, lu_precision):
+    """HBM bytes per launch of the BDF kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known size), when
+    that summary was taken on this workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "bdf_kernel_gri_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("cells") != ncells or rec.get("lu_precision") != lu_precision:
+        return None
+    return rec.get("traffic_bytes")
 
 
 def main():
@@ -72,6 +89,14 @@ def main():
         order = np.argsort(-y0[:, 0], kind="stable").astype(np.int32)
         d_order = torch.from_numpy(order).to(dev)
     stream = torch.cuda.current_stream(dev)
+    if a.calibrate_copy > 0:
+        nb = a.calibrate_copy << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {nb} bytes done", file=sys.stderr, flush=True)
+        del src, dst
     torch.cuda.synchronize()
 
     import ctypes as C
@@ -159,7 +184,7 @@ def main():
         flop_launch = work * n
         achieved = flop_launch / (ker32 / 1e3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(n, 1),
                 "flop_per_cell": work, "kernel_ms": ker32}
     out = {
         "metric": "GRI-3.0 cells integrated/sec at dt=1e-4 s, FP32-LU vs FP64 plain; p99 latency",

@@ -55,8 +55,45 @@ def _compile(src, flags, verbose):
     return obj
 
 
+HOST_DIR = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libhpc_ode_host.so")
+BIN = os.path.join(HERE, "bin")
+HOST_SOURCES = ["Mesh.cpp", "Integrator.cpp", "csvIO.cpp", "BDFIntegratorHIP.cpp"]
+GXX = shutil.which("g++") or "g++"
+HOST_CXX = ["-std=c++17", "-O2", "-Wall", "-fPIC", "-I", HOST_DIR, "-I", os.path.join(ROOT, "include")]
+ROCM_LINK = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+
+
+def _run(cmd, what, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{r.stdout}{r.stderr}")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ layer above the C ABI (Integrator / Mesh / CSV mirror of the reference's
+    interface): libhpc_ode_host.so, linked against libhpc_ode.so only through include/hpc_ode.h,
+    plus the hpo_integrate_csv tool and the C++ test program."""
+    os.makedirs(BIN, exist_ok=True)
+    srcs = [os.path.join(HOST_DIR, s) for s in HOST_SOURCES]
+    tmp = HOST_LIB + ".tmp"
+    _run([GXX, *HOST_CXX, "-shared", "-o", tmp, *srcs, "-L", HERE, "-lhpc_ode", "-Wl,-rpath,$ORIGIN",
+          *ROCM_LINK, "-ldl"], "linking libhpc_ode_host.so", verbose)
+    os.replace(tmp, HOST_LIB)
+    exe_link = ["-L", HERE, "-lhpc_ode_host", "-lhpc_ode", "-Wl,-rpath,$ORIGIN/..", *ROCM_LINK]
+    _run([GXX, *HOST_CXX, "-o", os.path.join(BIN, "hpo_integrate_csv"), os.path.join(HOST_DIR, "hpo_integrate_csv.cpp"),
+          *exe_link], "building hpo_integrate_csv", verbose)
+    test_src = os.path.join(ROOT, "tests", "cpp", "test_host.cpp")
+    if os.path.exists(test_src):
+        _run([GXX, *HOST_CXX, "-o", os.path.join(BIN, "test_host"), test_src, *exe_link], "building test_host", verbose)
+    return HOST_LIB
+
+
 def build(verbose: bool = False) -> str:
-    """Compiles every source and links libhpc_ode.so; returns its path."""
+    """Compiles every source and links libhpc_ode.so, then the host layer; returns the
+    path of libhpc_ode.so."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = [(s, f) for s, f in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
@@ -69,6 +106,8 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("linking libhpc_ode.so failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    if not os.environ.get("HPO_LIB"):
+        build_host(verbose)
     return LIB
 
 
