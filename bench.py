@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy-cells", type=int, default=4096,
+                    help="cells of the error statement against an rtol-1e-12 solution (0 = skip)")
     ap.add_argument("--profile", action="store_true", help="per-phase cycle counters (stderr)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -60,6 +62,37 @@ def measured_traffic(ncells, lu_precision):
     if rec.get("cells") != ncells or rec.get("lu_precision") != lu_precision:
         return None
     return rec.get("traffic_bytes")
+
+
+def _errors(y, truth):
+    """Max relative error in T and in mass fractions above 1e-10 (and above 1e-6)."""
+    eT = np.abs(y[:, 0] - truth[:, 0]) / truth[:, 0]
+    Yt, Y = truth[:, 1:], y[:, 1:]
+    rel = np.abs(Y - Yt) / np.maximum(np.abs(Yt), 1e-300)
+    return {"T_max_rel": float(eT.max()), "T_p99_rel": float(np.quantile(eT, 0.99)),
+            "Y_max_rel_above_1e-10": float(rel[np.abs(Yt) > 1e-10].max()),
+            "Y_max_rel_above_1e-6": float(rel[np.abs(Yt) > 1e-6].max())}
+
+
+def accuracy_statement(orc, y0, P, dt, y32, y64, ncells):
+    """Errors of the GPU results on an evenly spaced subset of the workload against the same
+    BDF run on the CPU at rtol 1e-12 / atol 1e-16, beside the CPU run at the production
+    tolerance on the same cells."""
+    idx = np.linspace(0, len(y0) - 1, ncells).astype(np.int64)
+    nth = min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    truth, st_t, _ = orc.integrate(P[idx], dt[idx], y0[idx], rtol=1e-12, atol=1e-16, maxsteps=10 ** 6,
+                                   nthreads=nth)
+    y6, st6, _ = orc.integrate(P[idx], dt[idx], y0[idx], nthreads=nth)
+    took = time.perf_counter() - t0
+    ok = st_t == 0
+    out = {"cells": int(ncells), "truth": "CPU BDF (same algorithm) rtol 1e-12 atol 1e-16",
+           "truth_failed": int((~ok).sum()), "cpu_seconds": took,
+           "fp32lu_gpu": _errors(y32[idx][ok], truth[ok]),
+           "cpu_rtol1e-6": _errors(y6[ok], truth[ok])}
+    if y64 is not None:
+        out["fp64_gpu"] = _errors(y64[idx][ok], truth[ok])
+    return out
 
 
 def main():
@@ -143,9 +176,11 @@ def main():
     ms32, ker32, st32, stats32, t32 = run(o32, a.steps, a.warmup)
     y32 = d_y.cpu().numpy()
     fp64 = None
+    y64 = None
     if not a.skip_fp64:
         o64 = H.opts(lu_precision=0, waves_per_block=a.waves_per_block, blocks_per_cu=a.blocks_per_cu)
         ms64, ker64, st64, stats64, _ = run(o64, a.steps, a.warmup)
+        y64 = d_y.cpu().numpy()
         fp64 = dict(ms_per_step=ms64, kernel_ms=ker64, cells_per_s=n / (ms64 / 1e3),
                     failed=int((st64 != 0).sum()))
 
@@ -178,6 +213,10 @@ def main():
     else:
         work = None
 
+    accuracy = None
+    if a.accuracy_cells > 0 and not a.no_cpu:
+        accuracy = accuracy_statement(orc, y0, P, dt, y32, y64, min(a.accuracy_cells, n))
+
     value = n / (ms32 / 1e3)
     roof = None
     if work is not None:
@@ -201,6 +240,7 @@ def main():
         "speedup_fp32lu_vs_fp64": (fp64["ms_per_step"] / ms32) if fp64 else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
         "work_model": {k: counts[k] for k in ("F_rhs", "T_rhs", "F_jac", "F_LU", "F_solve")} if counts else None,
     }
     print(json.dumps(out))

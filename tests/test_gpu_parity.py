@@ -160,3 +160,29 @@ def test_integrate_gri_fp32_lu(gri, sweeps):
     assert (sg == 0).all()
     _compare_states(yg, yo, 1e-5, 300.0)
     assert abs(stg["nst"].mean() - sto["nst"].mean()) <= 0.03 * sto["nst"].mean()
+
+
+def _rel_errors(y, truth):
+    eT = np.abs(y[:, 0] - truth[:, 0]) / truth[:, 0]
+    Yt = truth[:, 1:]
+    rel = np.abs(y[:, 1:] - Yt) / np.maximum(np.abs(Yt), 1e-300)
+    return eT.max(), rel[np.abs(Yt) > 1e-6].max()
+
+
+@pytest.mark.parametrize("lu_precision", [0, 1])
+def test_accuracy_vs_tight_reference(gri, lu_precision):
+    """Error against the rtol-1e-12 / atol-1e-16 solution of the same BDF (pinned against
+    atol 1e-20 and Radau in the CPU tests): the GPU result at rtol 1e-6 is as accurate as the
+    CPU oracle at rtol 1e-6, with FP64 LU and with FP32 LU + FP64 refinement."""
+    m, o = gri
+    y0, P, dt = WL.gri_plain(m.species, m.W, 64, seed=17)
+    truth, st, _ = o.integrate(P, dt, y0, rtol=1e-12, atol=1e-16, maxsteps=10 ** 6, nthreads=8)
+    assert (st == 0).all()
+    y6, _, _ = o.integrate(P, dt, y0, nthreads=8)
+    yg, sg, _ = m.integrate(y0, P, dt, lu_precision=lu_precision, refine_sweeps=1)
+    assert (sg == 0).all()
+    eT_g, eY_g = _rel_errors(yg, truth)
+    eT_o, eY_o = _rel_errors(y6, truth)
+    assert eT_g < 5e-4 and eY_g < 2e-2
+    assert eT_g <= 3 * eT_o + 1e-7
+    assert eY_g <= 3 * eY_o + 1e-5

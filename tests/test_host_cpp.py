@@ -108,7 +108,7 @@ def test_reference_e2e_100_cells(tmp_path, lu, gri_path):
     head, data = _read_csv(str(out))
     assert head[0] == "0CO0" and head[52] == "CO52" and head[53:] == ["ENTHA", "TEMPE", "Points:0", "Points:1",
                                                                      "Points:2"]
-    assert data.shape == (n, 59)
+    assert data.shape == (n, 58)
 
 
 @pytest.mark.gpu

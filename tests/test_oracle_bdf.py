@@ -122,3 +122,17 @@ def test_gri_cell_vs_radau(gri_path, T0):
     assert np.max(np.abs(yt[0] - ys)[m] / np.abs(ys)[m]) < 1e-5
     y6, st6, _ = g.integrate(P[0], dt, y0[None])
     assert abs(y6[0, 0] - ys[0]) / ys[0] < 1e-4
+
+
+def test_tight_reference_settings_are_converged(gri_path):
+    """The rtol-1e-12 / atol-1e-16 run used as the accuracy reference agrees with an
+    atol-1e-20 run to far below the rtol-1e-6 errors it is used to measure."""
+    from hpc_ode_amd import workloads as WL
+    g = O.Oracle(gri_path)
+    y0, P, dt = WL.gri_plain(g.mech.species, g.mech.W, 8, seed=3)
+    a, sa, _ = g.integrate(P, dt, y0, rtol=1e-12, atol=1e-16, maxsteps=10 ** 6, nthreads=8)
+    b, sb, _ = g.integrate(P, dt, y0, rtol=1e-12, atol=1e-20, maxsteps=10 ** 6, nthreads=8)
+    assert (sa == 0).all() and (sb == 0).all()
+    assert np.max(np.abs(a[:, 0] - b[:, 0]) / b[:, 0]) < 1e-9
+    m = np.abs(b[:, 1:]) > 1e-10
+    assert np.max(np.abs(a[:, 1:] - b[:, 1:])[m] / np.abs(b[:, 1:])[m]) < 1e-6
