@@ -36,7 +36,6 @@ SOURCES = [
     ("yaml_lite.cpp", HOST_FLAGS),
     ("mechanism.cpp", HOST_FLAGS),
     ("capi.cpp", HOST_FLAGS),
-    ("integrator.cpp", HOST_FLAGS),
     ("bdf_kernel.hip", HIP_FLAGS),
 ]
 

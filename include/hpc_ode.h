@@ -7,7 +7,7 @@
  * with CVodeIntegrator (src/Integrators/CVodeIntegrator.cpp:34-155) as its serial
  * implementation and CVodeIntegratorGPU (src/Integrators/CVodeIntegratorGPU.cpp:21-190)
  * as its GPU one. The functions below are what such an Integrator calls: the C++
- * class hpo::BDFIntegratorGPU (hpc_ode_amd/csrc/integrator.hpp) is the in-tree caller.
+ * class BDFIntegratorHIP (hpc_ode_amd/host/BDFIntegratorHIP.hpp) is the in-tree caller.
  *
  * Conventions: plain pointers and sizes; every function returns 0 on success or a
  * negative HPO_E* code, with a message available from hpo_last_error() (thread-local).
