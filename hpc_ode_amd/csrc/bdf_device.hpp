@@ -19,6 +19,7 @@ namespace dev {
 
 constexpr double kR = 8314.4621;
 constexpr double kPatm = 101325.0;
+constexpr double kLnPatmOverR = 2.5003367522806914;  // log(101325 / 8314.4621)
 
 // ---------------------------------------------------------------------------
 // cross-lane helpers (all lanes of the wave must be active)
@@ -51,6 +52,19 @@ HPO_DI double wsum(double v) {
   v += dpp<0x141>(v);
   v += dpp<0x140>(v);
   return (rl(v, 0) + rl(v, 16)) + (rl(v, 32) + rl(v, 48));
+}
+// Two sums at once; the interleaved chains hide each other's DPP latency.
+HPO_DI void wsum2(double& a, double& b) {
+  a += dpp<0xB1>(a);
+  b += dpp<0xB1>(b);
+  a += dpp<0x4E>(a);
+  b += dpp<0x4E>(b);
+  a += dpp<0x141>(a);
+  b += dpp<0x141>(b);
+  a += dpp<0x140>(a);
+  b += dpp<0x140>(b);
+  a = (rl(a, 0) + rl(a, 16)) + (rl(a, 32) + rl(a, 48));
+  b = (rl(b, 0) + rl(b, 16)) + (rl(b, 32) + rl(b, 48));
 }
 template <typename F>
 HPO_DI F wmax(F v) {
@@ -114,12 +128,14 @@ struct Tabs {
   const PdepRec* pd;       // [n_pdep]
   const double* effv;      // [n_eff]
   const uint8_t* effs;     // [n_eff]
+  const double* effpv;     // [n_pdep][neffp] padded efficiencies
+  const uint8_t* effps;    // [n_pdep][neffp] their species
   const double* coef;      // [16]
   const uint32_t* gather;  // [ngl][64]
   const uint16_t* pptr;    // [K+1]
   const uint16_t* slot_base;  // [R+1] (global)
   const uint64_t* jterm;      // [njl][64] (global)
-  int K, R, ngl, njl, npart;
+  int K, R, ngl, njl, npart, neffp, npdep;
 };
 
 HPO_DI Tabs make_tabs(const uint8_t* lds, const uint8_t* glob, const DevMechLayout& L) {
@@ -130,6 +146,10 @@ HPO_DI Tabs make_tabs(const uint8_t* lds, const uint8_t* glob, const DevMechLayo
   t.pd = reinterpret_cast<const PdepRec*>(lds + L.off_pdep);
   t.effv = reinterpret_cast<const double*>(lds + L.off_eff_val);
   t.effs = lds + L.off_eff_sp;
+  t.effpv = reinterpret_cast<const double*>(lds + L.off_effp_val);
+  t.effps = lds + L.off_effp_sp;
+  t.neffp = L.n_eff_pad;
+  t.npdep = L.n_pdep;
   t.coef = reinterpret_cast<const double*>(lds + L.off_coef);
   t.gather = reinterpret_cast<const uint32_t*>(lds + L.off_gather);
   t.pptr = reinterpret_cast<const uint16_t*>(lds + L.off_part_ptr);
@@ -202,18 +222,23 @@ HPO_DI void species_state(const Tabs& t, double y, double P, int lane, SpState& 
   s.invT = 1.0 / T;
   s.lnT = log(T);
   double Yk = shfl(y, (lane + 1) & 63);
-  double ysum = wsum((lane >= 1 && lane < K) ? y : 0.0);
-  if (lane == K - 1) Yk = 1.0 - ysum;
-  if (lane >= K) Yk = 0.0;
   const double* rec = t.sp + (lane < K ? lane : K - 1) * kSpStride;
   s.invW = rec[0];
   s.W = rec[1];
+  // sum Y_k (k < K-1) and sum Y_k/W_k (k < K-1) in one pair of interleaved reductions; the
+  // eliminated species adds (1 - sum Y) / W_{K-1}
+  double ys = (lane < K - 1) ? Yk : 0.0;
+  double yw = (lane < K - 1) ? Yk * s.invW : 0.0;
+  wsum2(ys, yw);
+  const double ylast = 1.0 - ys;
+  if (lane == K - 1) Yk = ylast;
+  if (lane >= K) Yk = 0.0;
   s.Yk = Yk;
-  double S = wsum(Yk * s.invW);
+  double S = yw + ylast * t.sp[(K - 1) * kSpStride];
   s.Wbar = 1.0 / S;
   s.rho = P * s.Wbar / (kR * T);
   s.m = P / (kR * T);
-  s.lnPfac = log(kPatm / (kR * T));
+  s.lnPfac = kLnPatmOverR - s.lnT;   // log(P_atm / (R T))
   const double* a = rec + 3 + (T <= rec[2] ? 0 : 7);
   s.cpR = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
   s.hRT = a[0] + T * (a[1] * 0.5 + T * (a[2] * (1.0 / 3.0) + T * (a[3] * 0.25 + T * a[4] * 0.2))) + a[5] * s.invT;
@@ -245,7 +270,7 @@ HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part
 #pragma unroll 4
   for (int i = 0; i < t.ngl; ++i) {
     uint32_t e = t.gather[i * 64 + lane];
-    double c = t.coef[(e >> 12) & 15u];
+    double c = static_cast<double>(static_cast<int32_t>(e << 16) >> 28);
     int r = static_cast<int>(e & 4095u);
     // flush the partial sum at the end of a species run; other entries write a scratch slot
     const bool flush = (e >> 16) & 1u;
@@ -271,6 +296,7 @@ HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part
 }
 
 // Net rate (fwd - rev) * pressure modification of reaction r (rxn_rates.c, rxn_rates_pres_mod.c).
+template <bool PDEP>
 HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, int r) {
   const RxnHdr h = t.hdr[r];
   const double* ar = t.arr + 3 * r;
@@ -293,10 +319,22 @@ HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, i
   // k_r = k_f / Kc, Kc = (P_atm/(R T))^dnu exp(-dG)
   double rev = (h.flags & 1) ? kf * exp(dG - dnu * s.lnPfac) * pc : 0.0;
   double pm = 1.0;
-  if (h.kind != kElementary) {
+  if (PDEP && h.kind != kElementary) {
     const PdepRec& pr = t.pd[h.pidx];
     double M = pr.eff_base * s.m;
-    for (int e2 = pr.eff_begin; e2 < pr.eff_end; ++e2) M += t.effv[e2] * w.C[t.effs[e2]];
+    // fixed-length padded list (padding: species with C = 1, efficiency 0): all loads of a
+    // group of four are independent, and the sum keeps the order of the real entries
+    const double* ev = t.effpv + h.pidx * t.neffp;
+    const uint8_t* es = t.effps + h.pidx * t.neffp;
+    for (int e2 = 0; e2 < t.neffp; e2 += 4) {
+      const uint32_t sp4 = *reinterpret_cast<const uint32_t*>(es + e2);
+      const double c0 = w.C[sp4 & 255u], c1 = w.C[(sp4 >> 8) & 255u];
+      const double c2 = w.C[(sp4 >> 16) & 255u], c3 = w.C[sp4 >> 24];
+      M += ev[e2] * c0;
+      M += ev[e2 + 1] * c1;
+      M += ev[e2 + 2] * c2;
+      M += ev[e2 + 3] * c3;
+    }
     if (h.kind == kThreeBody) {
       pm = M;
     } else {
@@ -304,13 +342,14 @@ HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, i
       double Pr = k0 * M / kf;
       double F = 1.0;
       if (h.kind == kTroe) {
-        double Fc = (1.0 - pr.troe_a) * exp(-s.T / pr.troe_T3) + pr.troe_a * exp(-s.T / pr.troe_T1) +
+        double Fc = (1.0 - pr.troe_a) * exp(-s.T * pr.troe_iT3) + pr.troe_a * exp(-s.T * pr.troe_iT1) +
                     ((h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0);
         double logFc = log10(fmax(Fc, 1.0e-300));
         double x = log10(fmax(Pr, 1.0e-300));
         double A = x - 0.67 * logFc - 0.4;
         double B = 0.806 - 1.1762 * logFc - 0.14 * x;
-        F = exp10(logFc / (1.0 + A * A / (B * B)));
+        const double B2 = B * B;
+        F = exp10(logFc * B2 / (B2 + A * A));   // logFc / (1 + A^2/B^2)
       }
       pm = F * Pr / (1.0 + Pr);
     }
@@ -334,14 +373,29 @@ HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane,
     w.g[lane] = 0.0;
   }
   wave_sync();
-  // two reactions per lane per pass: independent exp/log chains the scheduler can overlap
+  // Four reactions per lane per pass: independent exp chains the scheduler interleaves.
+  // Pressure-dependent reactions come first in the device order; with at most 64 of them
+  // only the first slot of the first pass can hold one, and the other slots take the
+  // elementary-only path (a mechanism with more takes the general path in every slot).
+  const bool few_pdep = t.npdep <= 64;
 #pragma unroll 1
-  for (int r = lane; r < R; r += 128) {
-    const int r2 = r + 64;
-    double p1 = reaction_rate(t, w, s, r);
-    double p2 = (r2 < R) ? reaction_rate(t, w, s, r2) : 0.0;
-    w.phi[r] = p1;
+  for (int base = 0; base < R; base += 256) {
+    const int r0 = base + lane, r1 = r0 + 64, r2 = r0 + 128, r3 = r0 + 192;
+    double p0 = (r0 < R) ? reaction_rate<true>(t, w, s, r0) : 0.0;
+    double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    if (few_pdep) {
+      p1 = (r1 < R) ? reaction_rate<false>(t, w, s, r1) : 0.0;
+      p2 = (r2 < R) ? reaction_rate<false>(t, w, s, r2) : 0.0;
+      p3 = (r3 < R) ? reaction_rate<false>(t, w, s, r3) : 0.0;
+    } else {
+      p1 = (r1 < R) ? reaction_rate<true>(t, w, s, r1) : 0.0;
+      p2 = (r2 < R) ? reaction_rate<true>(t, w, s, r2) : 0.0;
+      p3 = (r3 < R) ? reaction_rate<true>(t, w, s, r3) : 0.0;
+    }
+    if (r0 < R) w.phi[r0] = p0;
+    if (r1 < R) w.phi[r1] = p1;
     if (r2 < R) w.phi[r2] = p2;
+    if (r3 < R) w.phi[r3] = p3;
   }
   wave_sync();
   const double* src[1] = {w.phi};
@@ -447,10 +501,10 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
         double dPr_dM = k0 / kf;
         double qq = 1.0 / (1.0 + Pr);
         if (h.kind == kTroe) {
-          double e3 = exp(-s.T / pr.troe_T3), e1 = exp(-s.T / pr.troe_T1);
+          double e3 = exp(-s.T * pr.troe_iT3), e1 = exp(-s.T * pr.troe_iT1);
           double e2 = (h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0;
           double Fc = (1.0 - pr.troe_a) * e3 + pr.troe_a * e1 + e2;
-          double dFc = -(1.0 - pr.troe_a) / pr.troe_T3 * e3 - pr.troe_a / pr.troe_T1 * e1 +
+          double dFc = -(1.0 - pr.troe_a) * pr.troe_iT3 * e3 - pr.troe_a * pr.troe_iT1 * e1 +
                        ((h.flags & 4) ? pr.troe_T2 * s.invT * s.invT * e2 : 0.0);
           bool fcl = !(Fc > 1.0e-300), prl = !(Pr > 1.0e-300);
           double logFc = log10(fmax(Fc, 1.0e-300));

@@ -316,6 +316,10 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   }
   L.n_pdep = n_pdep;
   L.n_eff = n_eff;
+  int max_eff = 0;
+  for (int r = 0; r < R; ++r)
+    if (rx_of[r]->kind != kElementary) max_eff = std::max(max_eff, static_cast<int>(rx_of[r]->eff.size()));
+  L.n_eff_pad = align_to(std::max(max_eff, 1), 4);
 
   // balanced species gather over the nonzeros of the net stoichiometric matrix
   struct Nz {
@@ -325,7 +329,11 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   for (int k = 0; k < K; ++k)
     for (int r = 0; r < R; ++r) {
       auto it = net[r].find(k);
-      if (it != net[r].end() && it->second != 0.0) nz.push_back({k, r, coef_index(coef, it->second)});
+      if (it != net[r].end() && it->second != 0.0) {
+        const int nu = static_cast<int>(it->second);
+        if (nu < -8 || nu > 7) throw std::runtime_error("net stoichiometric coefficient out of range");
+        nz.push_back({k, r, nu});
+      }
     }
   const int NNZ = static_cast<int>(nz.size());
   const int per_lane = std::max(1, (NNZ + 63) / 64);
@@ -341,11 +349,11 @@ DevMechBlob build_device_tables(const Mechanism& m) {
         part = static_cast<uint32_t>(part_species.size());
         part_species.push_back(nz[t].k);
       }
-      uint32_t w = static_cast<uint32_t>(nz[t].r) | (static_cast<uint32_t>(nz[t].c) << 12) |
+      uint32_t w = static_cast<uint32_t>(nz[t].r) | ((static_cast<uint32_t>(nz[t].c) & 15u) << 12) |
                    (flush ? (1u << 16) : 0u) | (part << 17);
       gather[static_cast<size_t>(t - b) * 64 + lane] = w;
     }
-    // padding entries: r=0, coef 0 (0.0), no flush -> contribute nothing
+    // padding entries: r=0, nu 0, no flush -> contribute nothing
   }
   L.n_partials = static_cast<int>(part_species.size());
   if (L.n_partials > 32767) throw std::runtime_error("too many gather partials");
@@ -437,6 +445,11 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   off += n_eff * 8;
   L.off_eff_sp = off;
   off += align_to(std::max(n_eff, 1), 8);
+  off = align_to(off, 16);
+  L.off_effp_val = off;
+  off += n_pdep * L.n_eff_pad * 8;
+  L.off_effp_sp = off;
+  off += align_to(std::max(n_pdep * L.n_eff_pad, 1), 16);
   L.off_coef = off;
   off += 16 * 8;
   L.off_gather = off;
@@ -496,14 +509,21 @@ DevMechBlob build_device_tables(const Mechanism& m) {
         ++ie;
       }
       P.eff_end = ie;
+      double* pv = reinterpret_cast<double*>(B + L.off_effp_val) + ip * L.n_eff_pad;
+      uint8_t* ps = B + L.off_effp_sp + ip * L.n_eff_pad;
+      for (int j = 0; j < L.n_eff_pad; ++j) {
+        const bool real = j < static_cast<int>(rx.eff.size());
+        pv[j] = real ? rx.eff[j].second : 0.0;
+        ps[j] = real ? static_cast<uint8_t>(rx.eff[j].first) : kDummySpecies;
+      }
       bool constk0 = (rx.b0 == 0.0 && rx.theta0 == 0.0);
       if (rx.kind >= kLindemann && constk0) h.flags |= 8;
       P.lnA0 = (rx.kind >= kLindemann && constk0) ? rx.A0 : rx.lnA0;
       P.b0 = rx.b0;
       P.theta0 = rx.theta0;
       P.troe_a = rx.troe[0];
-      P.troe_T3 = rx.troe[1];
-      P.troe_T1 = rx.troe[2];
+      P.troe_iT3 = 1.0 / rx.troe[1];
+      P.troe_iT1 = 1.0 / rx.troe[2];
       P.troe_T2 = rx.troe[3];
       ++ip;
     }

@@ -67,7 +67,7 @@ struct alignas(8) PdepRec {     // per pressure-dependent reaction
   double eff_base;
   int32_t eff_begin, eff_end;   // range in eff tables
   double lnA0, b0, theta0;      // low-pressure limit (falloff)
-  double troe_a, troe_T3, troe_T1, troe_T2;
+  double troe_a, troe_iT3, troe_iT1, troe_T2;   // a, 1/T3, 1/T1 (0 -> inf: that term vanishes), T2
 };
 
 // Layout of the blob, offsets in bytes from its start. Sizes in elements.
@@ -78,6 +78,7 @@ struct DevMechLayout {
   int32_t n_slots;              // d(phi)/dC slots (Jacobian)
   int32_t n_jterm_lane;         // Jacobian assembly terms per lane
   int32_t n_coef;               // distinct stoichiometric coefficient values
+  int32_t n_eff_pad;            // efficiency entries per pressure-dependent reaction in the padded table
   int32_t lds_bytes;            // bytes [0, lds_bytes) are staged to LDS
   // LDS part
   int32_t off_species;          // double[K][16]: invW, W, tmid, (pad), nasa[14] -> see kSp*
@@ -86,8 +87,11 @@ struct DevMechLayout {
   int32_t off_pdep;             // PdepRec[n_pdep]
   int32_t off_eff_val;          // double[n_eff]
   int32_t off_eff_sp;           // uint8[n_eff] (padded to 8)
+  int32_t off_effp_val;         // double[n_pdep][n_eff_pad]: efficiencies, padded with 0.0
+  int32_t off_effp_sp;          // uint8[n_pdep][n_eff_pad]: their species, padded with kDummySpecies
   int32_t off_coef;             // double[16] distinct coefficient values
-  int32_t off_gather;           // uint32[n_gather_lane][64]  r | coef<<12 | flush<<16 | part<<17
+  int32_t off_gather;           // uint32[n_gather_lane][64]  r | nu<<12 | flush<<16 | part<<17
+                                //   (nu: net stoichiometric coefficient, 4-bit two's complement)
   int32_t off_part_ptr;         // uint16[K+1] partial-slot range per species
   // global part
   int32_t off_slot_base;        // uint16[R+1]: first d(phi)/dC slot of each reaction
