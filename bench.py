@@ -44,12 +44,17 @@ def parse():
     ap.add_argument("--accuracy-cells", type=int, default=4096,
                     help="cells of the error statement against an rtol-1e-12 solution (0 = skip)")
     ap.add_argument("--profile", action="store_true", help="per-phase cycle counters (stderr)")
+    ap.add_argument("--workload", choices=("plain", "uneven"), default="plain",
+                    help="plain: T0 U[1200,2000] K, dt 1e-4 s; uneven: T0 U[300,2500] K, phi log-U[0.3,3], "
+                         "dt log-U[1e-6,1e-3] s, launch shape and queue order from the autotuner")
+    ap.add_argument("--autotune", action="store_true", help="autotune the plain workload too")
+    ap.add_argument("--tune-sample", type=int, default=32768)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
-def measured_traffic(ncells, lu_precision):
+def measured_traffic(workload, ncells, lu_precision):
     """HBM bytes per launch of the BDF kernel from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known size), when
     that summary was taken on this workload; None otherwise."""
@@ -59,9 +64,17 @@ def measured_traffic(ncells, lu_precision):
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    if rec.get("cells") != ncells or rec.get("lu_precision") != lu_precision:
+    if rec.get("workload_kind") != workload or rec.get("cells") != ncells or rec.get("lu_precision") != lu_precision:
         return None
     return rec.get("traffic_bytes")
+
+
+def workload_name(kind, n):
+    cells = f"{n:,}"
+    if kind == "uneven":
+        return (f"GRI-3.0 {cells}-cell uneven substep (T0 U[300,2500] K, phi log-U[0.3,3], dt log-U[1e-6,1e-3] s), "
+                "FP32-LU + FP64 refinement, autotuned persistent kernel")
+    return f"GRI-3.0 {cells}-cell dt=1e-4 s substep, FP32-LU + FP64 refinement"
 
 
 def _errors(y, truth):
@@ -107,7 +120,11 @@ def main():
     from hpc_ode_amd import workloads as WL
 
     mech = H.Mechanism(MECH)
-    y0, P, dt = WL.gri_plain(mech.species, mech.W, a.cells, seed=a.seed)
+    if a.workload == "uneven":
+        y0, P, dt = WL.gri_uneven(mech.species, mech.W, a.cells, seed=a.seed)
+    else:
+        y0, P, dt = WL.gri_plain(mech.species, mech.W, a.cells, seed=a.seed)
+    tune = a.autotune or a.workload == "uneven"
     n, ns = y0.shape
     dev = torch.device("cuda:0")
     d_y0 = torch.from_numpy(y0).to(dev)
@@ -118,7 +135,7 @@ def main():
     d_stats = torch.empty((n, H.NSTATS), dtype=torch.int32, device=dev)
     d_prof = torch.zeros((n, H.NPROF), dtype=torch.int64, device=dev) if a.profile else None
     d_order = None
-    if a.sort:
+    if a.sort and not tune:
         order = np.argsort(-y0[:, 0], kind="stable").astype(np.int32)
         d_order = torch.from_numpy(order).to(dev)
     stream = torch.cuda.current_stream(dev)
@@ -173,12 +190,25 @@ def main():
 
     o32 = H.opts(lu_precision=1, refine_sweeps=a.refine_sweeps, waves_per_block=a.waves_per_block,
                  blocks_per_cu=a.blocks_per_cu)
+    tuned = None
+    if tune:
+        d_order = torch.empty(n, dtype=torch.int32, device=dev)
+        rep = H.TuneReport()
+        rc = H.lib().hpo_autotune(mech.handle, n, C.c_void_p(d_y0.data_ptr()), ns, C.c_void_p(d_P.data_ptr()),
+                                  C.c_void_p(d_dt.data_ptr()), a.tune_sample, C.byref(o32),
+                                  C.c_void_p(d_order.data_ptr()), C.byref(rep), C.c_void_p(stream.cuda_stream))
+        if rc != 0:
+            raise RuntimeError(H.lib().hpo_last_error().decode())
+        best = rep.candidates()[rep.best]
+        tuned = {"sample_cells": rep.sample_cells, "tune_ms": rep.tune_ms, "chosen": best,
+                 "candidates": rep.candidates()}
+        print(f"[bench] autotune: {best} in {rep.tune_ms:.0f} ms", file=sys.stderr, flush=True)
     ms32, ker32, st32, stats32, t32 = run(o32, a.steps, a.warmup)
     y32 = d_y.cpu().numpy()
     fp64 = None
     y64 = None
     if not a.skip_fp64:
-        o64 = H.opts(lu_precision=0, waves_per_block=a.waves_per_block, blocks_per_cu=a.blocks_per_cu)
+        o64 = H.opts(lu_precision=0, waves_per_block=o32.waves_per_block, blocks_per_cu=o32.blocks_per_cu)
         ms64, ker64, st64, stats64, _ = run(o64, a.steps, a.warmup)
         y64 = d_y.cpu().numpy()
         fp64 = dict(ms_per_step=ms64, kernel_ms=ker64, cells_per_s=n / (ms64 / 1e3),
@@ -215,7 +245,8 @@ def main():
 
     accuracy = None
     if a.accuracy_cells > 0 and not a.no_cpu:
-        accuracy = accuracy_statement(orc, y0, P, dt, y32, y64, min(a.accuracy_cells, n))
+        ncheck = min(a.accuracy_cells, n, 1024 if a.workload == "uneven" else n)
+        accuracy = accuracy_statement(orc, y0, P, dt, y32, y64, ncheck)
 
     value = n / (ms32 / 1e3)
     roof = None
@@ -223,15 +254,15 @@ def main():
         flop_launch = work * n
         achieved = flop_launch / (ker32 / 1e3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(n, 1),
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(a.workload, n, 1),
                 "flop_per_cell": work, "kernel_ms": ker32}
     out = {
         "metric": "GRI-3.0 cells integrated/sec at dt=1e-4 s, FP32-LU vs FP64 plain; p99 latency",
         "value": value, "unit": "cells/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 (FP32 Newton-matrix LU)", "data": "synthetic",
-        "config": {"workload": "GRI-3.0 1,048,576-cell dt=1e-4 s substep, FP32-LU + FP64 refinement"
-                   if n == (1 << 20) else f"GRI-3.0 {n}-cell dt=1e-4 s substep, FP32-LU",
+        "config": {"workload": workload_name(a.workload, n),
+                   "autotune": tuned,
                    "cells": n, "mechanism": "gri30.yaml (53 species, 325 reactions)",
                    "rtol": 1e-6, "atol": 1e-10, "refine_sweeps": a.refine_sweeps,
                    "grid": t32.grid, "block": t32.block, "lds_bytes": t32.lds_bytes},

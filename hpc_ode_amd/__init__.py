@@ -44,6 +44,22 @@ class Timing(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+TUNE_MAX = 16
+ORDER_MODES = {0: "cell index", 1: "T0 descending", 2: "predicted cost descending"}
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n_candidates", C.c_int32), ("best", C.c_int32),
+                ("waves_per_block", C.c_int32 * TUNE_MAX), ("blocks_per_cu", C.c_int32 * TUNE_MAX),
+                ("order_mode", C.c_int32 * TUNE_MAX), ("sample_ms", C.c_double * TUNE_MAX),
+                ("sample_cells", C.c_int64), ("tune_ms", C.c_double)]
+
+    def candidates(self):
+        return [dict(waves_per_block=self.waves_per_block[i], blocks_per_cu=self.blocks_per_cu[i],
+                     order=ORDER_MODES[self.order_mode[i]], sample_ms=self.sample_ms[i])
+                for i in range(self.n_candidates)]
+
+
 _lib = None
 
 
@@ -76,6 +92,9 @@ def lib():
         L.hpo_eval_jac.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
         L.hpo_lu_solve.argtypes = [C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip]
         L.hpo_device_info.argtypes = [_ip, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.hpo_autotune.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp, C.c_int64, C.POINTER(Opts), vp,
+                                   C.POINTER(TuneReport), vp]
+        L.hpo_batch_autotune.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(Opts), C.POINTER(TuneReport), vp]
         _lib = L
     return _lib
 
@@ -200,6 +219,13 @@ class Batch:
         _check(lib().hpo_batch_integrate(self._h, count, C.byref(o) if o is not None else None,
                                          stream, C.byref(t)))
         return t
+
+    def autotune(self, count, o: Opts, sample=32768, stream=None) -> TuneReport:
+        """Tunes launch shape and queue order for cells [0, count); o comes back tuned
+        (sort_cells = 2 selects the tuned order in integrate)."""
+        rep = TuneReport()
+        _check(lib().hpo_batch_autotune(self._h, count, sample, C.byref(o), C.byref(rep), stream))
+        return rep
 
     def get(self, count, first=0):
         n = self.mech.n_state

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -113,6 +114,7 @@ struct hpo_batch {
   int device = -1;
   double *d_y = nullptr, *d_P = nullptr, *d_dt = nullptr;
   int32_t *d_status = nullptr, *d_stats = nullptr, *d_order = nullptr;
+  int64_t tuned_count = -1;   // cells the order in d_order was tuned for (-1: none)
   ~hpo_batch() {
     if (device >= 0) {
       (void)hipSetDevice(device);
@@ -128,6 +130,7 @@ namespace {
 void check_opts(const hpo_opts& o) {
   if (!(o.rtol >= 0.0) || !(o.atol >= 0.0)) throw std::invalid_argument("tolerances must be >= 0");
   if (o.lu_precision != 0 && o.lu_precision != 1) throw std::invalid_argument("lu_precision must be 0 or 1");
+  if (o.sort_cells < 0 || o.sort_cells > 2) throw std::invalid_argument("sort_cells must be 0, 1 or 2");
   if (o.refine_sweeps < 0 || o.refine_sweeps > 8) throw std::invalid_argument("refine_sweeps must be in 0..8");
   if (o.waves_per_block < 0 || o.waves_per_block > 4) throw std::invalid_argument("waves_per_block must be in 0..4");
   if (o.blocks_per_cu < 0) throw std::invalid_argument("blocks_per_cu must be >= 0");
@@ -337,7 +340,10 @@ int hpo_batch_integrate(hpo_batch* b, int64_t count, const hpo_opts* o, void* st
     hpo_mech* m = const_cast<hpo_mech*>(b->mech);
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int32_t* order = nullptr;
-    if (opt.sort_cells && count > 1) {
+    if (opt.sort_cells == 2) {
+      if (b->tuned_count != count) throw std::invalid_argument("sort_cells = 2 needs hpo_batch_autotune for this count");
+      order = b->d_order;
+    } else if (opt.sort_cells == 1 && count > 1) {
       // hottest cells first: they take the most steps, so the queue tail holds short cells
       std::vector<double> T(count);
       HPO_HIP(hipMemcpy2D(T.data(), sizeof(double), b->d_y, b->ld * sizeof(double), sizeof(double), count,
@@ -346,6 +352,7 @@ int hpo_batch_integrate(hpo_batch* b, int64_t count, const hpo_opts* o, void* st
       std::iota(idx.begin(), idx.end(), 0);
       std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t c) { return T[a] > T[c]; });
       if (!b->d_order) b->d_order = dalloc<int32_t>(b->capacity);
+      b->tuned_count = -1;   // the tuned order is overwritten
       HPO_HIP(hipMemcpy(b->d_order, idx.data(), count * sizeof(int32_t), hipMemcpyHostToDevice));
       order = b->d_order;
     }
@@ -473,6 +480,190 @@ int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double* M, const dou
   }
   for (void* p : {static_cast<void*>(dM), static_cast<void*>(db), static_cast<void*>(dx), static_cast<void*>(dok)})
     if (p) (void)hipFree(p);
+  return rc;
+}
+
+namespace {
+
+// Mean step count per (T0, log10 dt) bin of the calibration sample; bins the sample left
+// empty take their temperature row's mean, then the overall mean.
+struct CostModel {
+  static constexpr int NT = 24, ND = 12;
+  double t0 = 0, t1 = 1, d0 = 0, d1 = 1;
+  double mean[NT][ND];
+  int bin_t(double T) const { return std::min(NT - 1, std::max(0, static_cast<int>((T - t0) / (t1 - t0) * NT))); }
+  int bin_d(double dt) const {
+    double l = std::log10(std::max(dt, 1e-300));
+    return std::min(ND - 1, std::max(0, static_cast<int>((l - d0) / (d1 - d0) * ND)));
+  }
+  void fit(const std::vector<double>& T, const std::vector<double>& dt, const std::vector<double>& cost) {
+    t0 = *std::min_element(T.begin(), T.end());
+    t1 = *std::max_element(T.begin(), T.end()) + 1e-9;
+    d0 = std::log10(std::max(*std::min_element(dt.begin(), dt.end()), 1e-300));
+    d1 = std::log10(std::max(*std::max_element(dt.begin(), dt.end()), 1e-300)) + 1e-9;
+    double sum[NT][ND] = {}, cnt[NT][ND] = {};
+    double all = 0;
+    for (size_t i = 0; i < T.size(); ++i) {
+      sum[bin_t(T[i])][bin_d(dt[i])] += cost[i];
+      cnt[bin_t(T[i])][bin_d(dt[i])] += 1;
+      all += cost[i];
+    }
+    all /= std::max<size_t>(T.size(), 1);
+    for (int a = 0; a < NT; ++a) {
+      double rs = 0, rc = 0;
+      for (int b = 0; b < ND; ++b) rs += sum[a][b], rc += cnt[a][b];
+      for (int b = 0; b < ND; ++b)
+        mean[a][b] = cnt[a][b] > 0 ? sum[a][b] / cnt[a][b] : (rc > 0 ? rs / rc : all);
+    }
+  }
+  double operator()(double T, double dt) const { return mean[bin_t(T)][bin_d(dt)]; }
+};
+
+// Order of cells by a key, descending, ties by index.
+std::vector<int32_t> order_desc(const std::vector<double>& key) {
+  std::vector<int32_t> idx(key.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return key[a] > key[b]; });
+  return idx;
+}
+
+}  // namespace
+
+int hpo_autotune(const hpo_mech* mc, int64_t ncells, const double* d_y, int64_t ld, const double* d_P,
+                 const double* d_dt, int64_t sample_cells, hpo_opts* o, int32_t* d_order_out,
+                 hpo_tune_report* rep, void* stream) {
+  if (!mc || !d_y || !d_P || !d_dt || !o || !d_order_out || ncells <= 0) return fail(HPO_E_INVALID, "bad argument");
+  hpo_mech* m = const_cast<hpo_mech*>(mc);
+  const auto wall0 = std::chrono::steady_clock::now();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double *sy = nullptr, *sP = nullptr, *sdt = nullptr, *syo = nullptr, *dT0 = nullptr;
+  int32_t *sst = nullptr, *sstats = nullptr, *sorder = nullptr;
+  int rc = HPO_OK;
+  try {
+    check_opts(*o);
+    m->ensure_device();
+    const int64_t ns = m->blob.lay.K;
+    if (ld < ns) throw std::invalid_argument("leading dimension smaller than the state");
+    // evenly strided sample
+    const int64_t ns_req = sample_cells > 0 ? sample_cells : 32768;
+    const int64_t stride = std::max<int64_t>(1, ncells / std::max<int64_t>(1, ns_req));
+    const int64_t S = std::min<int64_t>(ns_req, (ncells + stride - 1) / stride);
+    sy = dalloc<double>(static_cast<size_t>(S * ns));
+    syo = dalloc<double>(static_cast<size_t>(S * ns));
+    sP = dalloc<double>(S);
+    sdt = dalloc<double>(S);
+    sst = dalloc<int32_t>(S);
+    sstats = dalloc<int32_t>(static_cast<size_t>(S * HPO_NSTATS));
+    sorder = dalloc<int32_t>(S);
+    HPO_HIP(hipMemcpy2DAsync(sy, ns * sizeof(double), d_y, stride * ld * sizeof(double), ns * sizeof(double), S,
+                             hipMemcpyDeviceToDevice, st));
+    HPO_HIP(hipMemcpy2DAsync(sP, sizeof(double), d_P, stride * sizeof(double), sizeof(double), S,
+                             hipMemcpyDeviceToDevice, st));
+    HPO_HIP(hipMemcpy2DAsync(sdt, sizeof(double), d_dt, stride * sizeof(double), sizeof(double), S,
+                             hipMemcpyDeviceToDevice, st));
+    // T0 and dt of every cell (host), for the orders of the full workload
+    // (the strided temperature column is packed on the device first: a strided copy straight
+    // to the host moves one 8-byte row per transfer)
+    std::vector<double> T(ncells), dt(ncells);
+    dT0 = dalloc<double>(ncells);
+    HPO_HIP(hipMemcpy2DAsync(dT0, sizeof(double), d_y, ld * sizeof(double), sizeof(double), ncells,
+                             hipMemcpyDeviceToDevice, st));
+    HPO_HIP(hipMemcpyAsync(T.data(), dT0, ncells * sizeof(double), hipMemcpyDeviceToHost, st));
+    HPO_HIP(hipMemcpyAsync(dt.data(), d_dt, ncells * sizeof(double), hipMemcpyDeviceToHost, st));
+    HPO_HIP(hipStreamSynchronize(st));
+    std::vector<double> sT(S), sdtv(S);
+    for (int64_t i = 0; i < S; ++i) sT[i] = T[i * stride], sdtv[i] = dt[i * stride];
+
+    // calibration run: step counts of the sample cells
+    hpo_opts base = *o;
+    base.waves_per_block = 0;
+    base.blocks_per_cu = 0;
+    base.sort_cells = 0;
+    hpo_timing tm{};
+    launch_integrate(m, S, sy, syo, ns, sP, sdt, sst, sstats, nullptr, nullptr, base, st, &tm);
+    std::vector<int32_t> stats(static_cast<size_t>(S * HPO_NSTATS));
+    HPO_HIP(hipMemcpy(stats.data(), sstats, stats.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<double> cost(S);
+    for (int64_t i = 0; i < S; ++i) cost[i] = stats[i * HPO_NSTATS + 1];   // nfe: RHS evaluations
+    CostModel cm;
+    cm.fit(sT, sdtv, cost);
+
+    // sample orders for the three modes
+    std::vector<std::vector<int32_t>> sample_orders(3);
+    sample_orders[0].resize(S);
+    std::iota(sample_orders[0].begin(), sample_orders[0].end(), 0);
+    sample_orders[1] = order_desc(sT);
+    std::vector<double> pred(S);
+    for (int64_t i = 0; i < S; ++i) pred[i] = cm(sT[i], sdtv[i]);
+    sample_orders[2] = order_desc(pred);
+
+    hpo_tune_report R{};
+    R.sample_cells = S;
+    const int wpbs[3] = {4, 2, 1};
+    double best_ms = 1e300;
+    int best = -1;
+    for (int wi = 0; wi < 3; ++wi)
+      for (int om = 0; om < 3 && R.n_candidates < HPO_TUNE_MAX; ++om) {
+        HPO_HIP(hipMemcpyAsync(sorder, sample_orders[om].data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        hpo_opts c = base;
+        c.waves_per_block = wpbs[wi];
+        hpo_timing t{};
+        launch_integrate(m, S, sy, syo, ns, sP, sdt, sst, nullptr, sorder, nullptr, c, st, &t);
+        const int k = R.n_candidates++;
+        R.waves_per_block[k] = wpbs[wi];
+        R.blocks_per_cu[k] = (t.grid > 0 && m->n_cu > 0) ? std::max(1, t.grid / m->n_cu) : 0;
+        R.order_mode[k] = om;
+        R.sample_ms[k] = t.kernel_ms;
+        if (t.kernel_ms < best_ms) best_ms = t.kernel_ms, best = k;
+      }
+    R.best = best;
+    o->waves_per_block = R.waves_per_block[best];
+    o->blocks_per_cu = 0;   // as many workgroups per CU as fit (what every candidate ran with)
+    o->sort_cells = 0;
+
+    // the chosen order over the whole workload
+    std::vector<int32_t> full;
+    if (R.order_mode[best] == 0) {
+      full.resize(ncells);
+      std::iota(full.begin(), full.end(), 0);
+    } else if (R.order_mode[best] == 1) {
+      full = order_desc(T);
+    } else {
+      std::vector<double> p(ncells);
+      for (int64_t i = 0; i < ncells; ++i) p[i] = cm(T[i], dt[i]);
+      full = order_desc(p);
+    }
+    HPO_HIP(hipMemcpyAsync(d_order_out, full.data(), ncells * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HPO_HIP(hipStreamSynchronize(st));
+    R.tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    if (rep) *rep = R;
+  } catch (...) {
+    rc = map_exceptions();
+  }
+  (void)hipStreamSynchronize(st);
+  for (void* p : {static_cast<void*>(sy), static_cast<void*>(syo), static_cast<void*>(sP), static_cast<void*>(sdt),
+                  static_cast<void*>(sst), static_cast<void*>(sstats), static_cast<void*>(sorder),
+                  static_cast<void*>(dT0)})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+int hpo_batch_autotune(hpo_batch* b, int64_t count, int64_t sample_cells, hpo_opts* o, hpo_tune_report* rep,
+                       void* stream) {
+  if (!b || !o) return fail(HPO_E_INVALID, "null argument");
+  if (count <= 0 || count > b->capacity) return fail(HPO_E_INVALID, "count out of range");
+  try {
+    HPO_HIP(hipSetDevice(b->device));
+    if (!b->d_order) b->d_order = dalloc<int32_t>(b->capacity);
+  } catch (...) {
+    return map_exceptions();
+  }
+  b->tuned_count = -1;
+  int rc = hpo_autotune(b->mech, count, b->d_y, b->ld, b->d_P, b->d_dt, sample_cells, o, b->d_order, rep, stream);
+  if (rc == HPO_OK) {
+    b->tuned_count = count;
+    o->sort_cells = 2;
+  }
   return rc;
 }
 

@@ -53,7 +53,8 @@ typedef struct {
   int32_t refine_sweeps;   /* refinement sweeps per linear solve when lu_precision == 1 */
   int32_t waves_per_block; /* cells resident per workgroup (1..4); 0 = default */
   int32_t blocks_per_cu;   /* persistent workgroups per CU; 0 = as many as fit */
-  int32_t sort_cells;      /* nonzero: work queue ordered by descending initial temperature */
+  int32_t sort_cells;      /* work-queue order: 0 cell index; 1 descending initial temperature;
+                              2 (hpo_batch_integrate only) the order of the last hpo_batch_autotune */
   int32_t reserved[2];
 } hpo_opts;
 
@@ -104,6 +105,33 @@ int hpo_integrate_device(const hpo_mech *m, int64_t ncells, const double *d_y_in
                          int64_t ld, const double *d_P, const double *d_dt, int32_t *d_status,
                          int32_t *d_stats, const int32_t *d_order, int64_t *d_prof, const hpo_opts *o,
                          void *hip_stream, hpo_timing *timing);
+
+/* ---- autotuner for uneven workloads (persistent kernel + work queue) ----
+ * No counterpart in the reference (its GPU path integrates one batch with shared steps,
+ * src/Integrators/CVodeIntegratorGPU.cpp:60-108). Times launch shapes and work-queue orders on
+ * an evenly strided sample of the workload (integrated into scratch buffers: the inputs are not
+ * modified) and returns the fastest.
+ * Orders: 0 = cell index, 1 = initial temperature descending, 2 = predicted cost descending
+ * (cost model: mean BDF step count of the sample cells per (T0, log dt) bin).
+ * o: in = tolerances and LU precision to tune for; out = waves_per_block, blocks_per_cu set,
+ * sort_cells cleared. d_order_out: [ncells] device array, receives the chosen order (a
+ * permutation of 0..ncells-1) to pass to hpo_integrate_device. */
+#define HPO_TUNE_MAX 16
+typedef struct {
+  int32_t n_candidates, best;
+  int32_t waves_per_block[HPO_TUNE_MAX], blocks_per_cu[HPO_TUNE_MAX], order_mode[HPO_TUNE_MAX];
+  double sample_ms[HPO_TUNE_MAX];
+  int64_t sample_cells;
+  double tune_ms;          /* wall time of the whole tuning call */
+} hpo_tune_report;
+int hpo_autotune(const hpo_mech *m, int64_t ncells, const double *d_y, int64_t ld, const double *d_P,
+                 const double *d_dt, int64_t sample_cells, hpo_opts *o, int32_t *d_order_out,
+                 hpo_tune_report *report, void *hip_stream);
+
+/* The same tuning on a resident batch (cells [0, count)). The order is kept in the batch and
+ * used by hpo_batch_integrate when o->sort_cells == 2; o comes back with sort_cells = 2. */
+int hpo_batch_autotune(hpo_batch *b, int64_t count, int64_t sample_cells, hpo_opts *o,
+                       hpo_tune_report *report, void *hip_stream);
 
 /* ---- component entry points (host arrays), used by parity tests ---- */
 /* f = dydt(y) for n states: y, f are [n][n_state]; the pyJac dydt of dydt.c:7-133. */
