@@ -60,6 +60,23 @@ class TuneReport(C.Structure):
                 for i in range(self.n_candidates)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("n_slots", C.c_int32), ("reserved0", C.c_int32), ("max_batch_cells", C.c_int64),
+                ("hbm_budget_bytes", C.c_int64), ("opts", Opts)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("latency_ms", C.c_double), ("queue_ms", C.c_double), ("run_ms", C.c_double),
+                ("ncells", C.c_int64), ("batch_cells", C.c_int64), ("batch_requests", C.c_int32),
+                ("failed_cells", C.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+E_BUSY = -6
+E_TIMEOUT = -7
+
 _lib = None
 
 
@@ -95,6 +112,16 @@ def lib():
         L.hpo_autotune.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp, C.c_int64, C.POINTER(Opts), vp,
                                    C.POINTER(TuneReport), vp]
         L.hpo_batch_autotune.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(Opts), C.POINTER(TuneReport), vp]
+        L.hpo_server_opts_default.argtypes = [C.POINTER(ServerOpts)]
+        L.hpo_server_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(ServerOpts), C.POINTER(vp)]
+        L.hpo_server_destroy.argtypes = [vp]
+        L.hpo_server_open_stream.argtypes = [vp, C.c_int32, C.c_int64, _dp, C.c_int64, _dp, C.c_double,
+                                             C.POINTER(C.c_int64)]
+        L.hpo_server_close_stream.argtypes = [vp, C.c_int64]
+        L.hpo_server_submit.argtypes = [vp, C.c_int64, C.c_double, _dp, C.c_int64, _ip, C.POINTER(C.c_int64)]
+        L.hpo_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(RequestInfo)]
+        L.hpo_server_stream_get.argtypes = [vp, C.c_int64, _dp, C.c_int64]
+        L.hpo_server_hbm_used.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -234,6 +261,74 @@ class Batch:
         stats = np.zeros((count, NSTATS), np.int32)
         _check(lib().hpo_batch_get(self._h, first, count, _p(y), n, _p(st, _ip), _p(stats, _ip)))
         return y, st, {f: stats[:, i].copy() for i, f in enumerate(STATS_FIELDS)}
+
+
+class Server:
+    """Serving loop (hpo_server): resident cell streams, substep requests, continuous batching."""
+
+    def __init__(self, mechs, n_slots=4, max_batch_cells=262144, hbm_budget_bytes=0, **opt_kw):
+        so = ServerOpts()
+        lib().hpo_server_opts_default(C.byref(so))
+        so.n_slots, so.max_batch_cells, so.hbm_budget_bytes = n_slots, max_batch_cells, hbm_budget_bytes
+        so.opts = opts(**opt_kw)
+        self.mechs = list(mechs)
+        arr = (C.c_void_p * len(self.mechs))(*[m.handle.value for m in self.mechs])
+        h = C.c_void_p()
+        _check(lib().hpo_server_create(arr, len(self.mechs), C.byref(so), C.byref(h)))
+        self._h = h
+        self._pending = {}   # request id -> (y_out, status_out): keeps the buffers alive
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hpo_server_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def open_stream(self, mech_index, y0, P):
+        """Returns a stream id, or None when the HBM budget is exhausted."""
+        y0 = np.ascontiguousarray(np.atleast_2d(y0), np.float64)
+        n = y0.shape[0]
+        Pa = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
+        sid = C.c_int64()
+        rc = lib().hpo_server_open_stream(self._h, mech_index, n, _p(y0), y0.shape[1], _p(Pa), 0.0, C.byref(sid))
+        if rc == E_BUSY:
+            return None
+        _check(rc)
+        return sid.value
+
+    def close_stream(self, sid):
+        _check(lib().hpo_server_close_stream(self._h, sid))
+
+    def submit(self, sid, dt, want_state=True, n=None, n_state=None):
+        """Enqueues one substep; returns a request id. With want_state the result rows and
+        statuses are delivered to wait()."""
+        y = st = None
+        if want_state:
+            y = np.zeros((n, n_state))
+            st = np.zeros(n, np.int32)
+        rid = C.c_int64()
+        _check(lib().hpo_server_submit(self._h, sid, dt, _p(y) if y is not None else None,
+                                       n_state if y is not None else 0, _p(st, _ip) if st is not None else None,
+                                       C.byref(rid)))
+        self._pending[rid.value] = (y, st)
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        info = RequestInfo()
+        _check(lib().hpo_server_wait(self._h, rid, timeout, C.byref(info)))
+        y, st = self._pending.pop(rid)
+        return y, st, info.as_dict()
+
+    def stream_state(self, sid, n, n_state):
+        y = np.zeros((n, n_state))
+        _check(lib().hpo_server_stream_get(self._h, sid, _p(y), n_state))
+        return y
+
+    def hbm(self):
+        u, b = C.c_int64(), C.c_int64()
+        _check(lib().hpo_server_hbm_used(self._h, C.byref(u), C.byref(b)))
+        return u.value, b.value
 
 
 def lu_solve(M, b, fp32=False):

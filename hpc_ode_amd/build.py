@@ -36,6 +36,7 @@ SOURCES = [
     ("yaml_lite.cpp", HOST_FLAGS),
     ("mechanism.cpp", HOST_FLAGS),
     ("capi.cpp", HOST_FLAGS),
+    ("server.cpp", HOST_FLAGS),
     ("bdf_kernel.hip", HIP_FLAGS),
 ]
 
@@ -98,7 +99,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
         objs = list(ex.map(lambda sf: _compile(sf[0], sf[1], verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-pthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -13,10 +13,10 @@
 #include <string>
 #include <vector>
 
-#include "kernels.hpp"
-#include "mechanism.hpp"
+#include "capi_internal.hpp"
 
-namespace {
+namespace hpo {
+namespace capi {
 
 thread_local std::string g_err;
 
@@ -24,18 +24,6 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
-
-struct HipError : std::runtime_error {
-  hipError_t e;
-  HipError(hipError_t err, const std::string& what) : std::runtime_error(what), e(err) {}
-};
-
-#define HPO_HIP(call)                                                                               \
-  do {                                                                                              \
-    hipError_t _e = (call);                                                                         \
-    if (_e != hipSuccess)                                                                           \
-      throw HipError(_e, std::string(#call) + ": " + hipGetErrorString(_e));                        \
-  } while (0)
 
 int map_exceptions() {
   try {
@@ -51,61 +39,43 @@ int map_exceptions() {
   }
 }
 
-template <typename T>
-T* dalloc(size_t n) {
-  void* p = nullptr;
-  HPO_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
-  return static_cast<T*>(p);
+}  // namespace capi
+}  // namespace hpo
+
+using namespace hpo::capi;
+
+hpo_mech::~hpo_mech() {
+  if (device >= 0) {
+    (void)hipSetDevice(device);
+    if (d_blob) (void)hipFree(d_blob);
+    if (d_scratch) (void)hipFree(d_scratch);
+    if (d_queue) (void)hipFree(d_queue);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
 }
 
-}  // namespace
+void hpo_mech::ensure_device() {
+  if (d_blob) return;
+  HPO_HIP(hipGetDevice(&device));
+  hipDeviceProp_t prop;
+  HPO_HIP(hipGetDeviceProperties(&prop, device));
+  n_cu = prop.multiProcessorCount;
+  d_blob = dalloc<uint8_t>(blob.bytes.size());
+  HPO_HIP(hipMemcpy(d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice));
+  d_queue = dalloc<uint32_t>(4);
+  HPO_HIP(hipEventCreate(&ev0));
+  HPO_HIP(hipEventCreate(&ev1));
+  scratch_stride = hpo::wave_scratch_doubles_host(blob.lay);
+}
 
-struct hpo_mech {
-  hpo::Mechanism host;
-  hpo::DevMechBlob blob;
-  int device = -1;
-  int n_cu = 0;
-  uint8_t* d_blob = nullptr;
-  double* d_scratch = nullptr;
-  int64_t scratch_slots = 0;
-  int64_t scratch_stride = 0;
-  uint32_t* d_queue = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
-  ~hpo_mech() {
-    if (device >= 0) {
-      (void)hipSetDevice(device);
-      if (d_blob) (void)hipFree(d_blob);
-      if (d_scratch) (void)hipFree(d_scratch);
-      if (d_queue) (void)hipFree(d_queue);
-      if (ev0) (void)hipEventDestroy(ev0);
-      if (ev1) (void)hipEventDestroy(ev1);
-    }
-  }
-
-  // Uploads the tables on first use (on the current device).
-  void ensure_device() {
-    if (d_blob) return;
-    HPO_HIP(hipGetDevice(&device));
-    hipDeviceProp_t prop;
-    HPO_HIP(hipGetDeviceProperties(&prop, device));
-    n_cu = prop.multiProcessorCount;
-    d_blob = dalloc<uint8_t>(blob.bytes.size());
-    HPO_HIP(hipMemcpy(d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice));
-    d_queue = dalloc<uint32_t>(4);
-    HPO_HIP(hipEventCreate(&ev0));
-    HPO_HIP(hipEventCreate(&ev1));
-    scratch_stride = hpo::wave_scratch_doubles_host(blob.lay);
-  }
-
-  void ensure_scratch(int64_t slots) {
-    if (slots <= scratch_slots) return;
-    if (d_scratch) HPO_HIP(hipFree(d_scratch));
-    d_scratch = nullptr;
-    d_scratch = dalloc<double>(static_cast<size_t>(slots * scratch_stride));
-    scratch_slots = slots;
-  }
-};
+void hpo_mech::ensure_scratch(int64_t slots) {
+  if (slots <= scratch_slots) return;
+  if (d_scratch) HPO_HIP(hipFree(d_scratch));
+  d_scratch = nullptr;
+  d_scratch = dalloc<double>(static_cast<size_t>(slots * scratch_stride));
+  scratch_slots = slots;
+}
 
 struct hpo_batch {
   const hpo_mech* mech = nullptr;
@@ -125,7 +95,8 @@ struct hpo_batch {
   }
 };
 
-namespace {
+namespace hpo {
+namespace capi {
 
 void check_opts(const hpo_opts& o) {
   if (!(o.rtol >= 0.0) || !(o.atol >= 0.0)) throw std::invalid_argument("tolerances must be >= 0");
@@ -140,7 +111,7 @@ void check_opts(const hpo_opts& o) {
 void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* d_yout, int64_t ld,
                       const double* d_P, const double* d_dt, int32_t* d_status, int32_t* d_stats,
                       const int32_t* d_order, int64_t* d_prof, const hpo_opts& o, hipStream_t st,
-                      hpo_timing* tm) {
+                      hpo_timing* tm, const ExecRes* res) {
   check_opts(o);
   const int ns = m->blob.lay.K;
   if (!hpo::kernel_supports(ns))
@@ -160,7 +131,17 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   int64_t grid = static_cast<int64_t>(m->n_cu) * bpc;
   grid = std::min<int64_t>(grid, (ncells + wpb - 1) / wpb);
   grid = std::max<int64_t>(grid, 1);
-  m->ensure_scratch(grid * wpb);
+  ExecRes own;
+  if (!res) {
+    m->ensure_scratch(grid * wpb);
+    own.d_queue = m->d_queue;
+    own.d_scratch = m->d_scratch;
+    own.scratch_slots = m->scratch_slots;
+    own.scratch_stride = m->scratch_stride;
+    res = &own;
+  }
+  if (res->scratch_slots < grid * wpb || res->scratch_stride < m->scratch_stride)
+    throw std::invalid_argument("launch context scratch too small for this launch");
 
   hpo::BdfArgs A{};
   A.mech = m->d_blob;
@@ -172,9 +153,9 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   A.status = d_status;
   A.stats = d_stats;
   A.order = d_order;
-  A.queue = m->d_queue;
-  A.scratch = m->d_scratch;
-  A.scratch_stride = m->scratch_stride;
+  A.queue = res->d_queue;
+  A.scratch = res->d_scratch;
+  A.scratch_stride = res->scratch_stride;
   A.ncells = ncells;
   A.ld = ld;
   A.rtol = o.rtol;
@@ -183,10 +164,10 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   A.refine_sweeps = o.refine_sweeps;
   A.prof = reinterpret_cast<int64_t*>(d_prof);
 
-  HPO_HIP(hipMemsetAsync(m->d_queue, 0, sizeof(uint32_t), st));
-  HPO_HIP(hipEventRecord(m->ev0, st));
+  HPO_HIP(hipMemsetAsync(res->d_queue, 0, sizeof(uint32_t), st));
+  if (tm) HPO_HIP(hipEventRecord(m->ev0, st));
   if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, o.lu_precision, static_cast<int>(grid), block, lds, st));
-  HPO_HIP(hipEventRecord(m->ev1, st));
+  if (tm) HPO_HIP(hipEventRecord(m->ev1, st));
   if (tm) {
     HPO_HIP(hipEventSynchronize(m->ev1));
     float ms = 0.f;
@@ -199,7 +180,25 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   }
 }
 
-}  // namespace
+int64_t max_wave_slots(hpo_mech* m) {
+  m->ensure_device();
+  const int ns = m->blob.lay.K;
+  int64_t most = 1;
+  // one-wave workgroups pack the most waves onto a CU; the launch grid never exceeds
+  // n_CU x (workgroups per CU) x (waves per workgroup)
+  for (int wpb = 1; wpb <= 4; ++wpb)
+    for (int lu = 0; lu < 2; ++lu) {
+      const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) +
+                         static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
+      int occ = 0;
+      HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_kernel_fn(ns, lu), 64 * wpb, lds));
+      most = std::max<int64_t>(most, static_cast<int64_t>(occ) * wpb);
+    }
+  return static_cast<int64_t>(m->n_cu) * most;
+}
+
+}  // namespace capi
+}  // namespace hpo
 
 extern "C" {
 

@@ -1,0 +1,80 @@
+// Internals shared by the C ABI implementation files (capi.cpp, server.cpp): error
+// plumbing, device allocation, the mechanism handle and the launch of the BDF kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/hpc_ode.h"
+#include "kernels.hpp"
+#include "mechanism.hpp"
+
+namespace hpo {
+namespace capi {
+
+int fail(int code, const std::string& msg);
+int map_exceptions();   // call inside a catch(...) block
+
+struct HipError : std::runtime_error {
+  hipError_t e;
+  HipError(hipError_t err, const std::string& what) : std::runtime_error(what), e(err) {}
+};
+
+#define HPO_HIP(call)                                                                               \
+  do {                                                                                              \
+    hipError_t _e = (call);                                                                         \
+    if (_e != hipSuccess)                                                                           \
+      throw ::hpo::capi::HipError(_e, std::string(#call) + ": " + hipGetErrorString(_e));           \
+  } while (0)
+
+template <typename T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  HPO_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+// Work-queue counter and per-wave scratch of one launch context. Launches that may run at the
+// same time need separate contexts; a mechanism handle owns one for its own entry points.
+struct ExecRes {
+  uint32_t* d_queue = nullptr;
+  double* d_scratch = nullptr;
+  int64_t scratch_slots = 0;    // wave slots the scratch holds
+  int64_t scratch_stride = 0;   // doubles per wave slot
+};
+
+void check_opts(const hpo_opts& o);
+
+// Launches the persistent BDF kernel on arrays already resident on the device. With res ==
+// nullptr the mechanism's own context is used (and grown as needed); an explicit context must
+// hold the grid's wave slots already. With tm != nullptr the call waits and reports timing.
+void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* d_yout, int64_t ld,
+                      const double* d_P, const double* d_dt, int32_t* d_status, int32_t* d_stats,
+                      const int32_t* d_order, int64_t* d_prof, const hpo_opts& o, hipStream_t st,
+                      hpo_timing* tm, const ExecRes* res = nullptr);
+
+// Wave slots a launch on the whole device can use (scratch sizing for explicit contexts).
+int64_t max_wave_slots(hpo_mech* m);
+
+}  // namespace capi
+}  // namespace hpo
+
+struct hpo_mech {
+  hpo::Mechanism host;
+  hpo::DevMechBlob blob;
+  int device = -1;
+  int n_cu = 0;
+  uint8_t* d_blob = nullptr;
+  double* d_scratch = nullptr;
+  int64_t scratch_slots = 0;
+  int64_t scratch_stride = 0;
+  uint32_t* d_queue = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  ~hpo_mech();
+  void ensure_device();               // uploads the tables on first use (current device)
+  void ensure_scratch(int64_t slots);
+};

@@ -1,0 +1,460 @@
+// Serving loop (include/hpc_ode.h, hpo_server_*): continuous batching of chemistry-substep
+// requests over device-resident cell streams.
+//
+// Device memory: every stream owns its rows y [n][K], pressures P [n] and statuses [n] from
+// open to close (admitted against the HBM budget). Every launch slot owns a HIP stream, a
+// work-queue counter, per-wave scratch sized for a whole-device launch of any of the server's
+// mechanisms, and packing buffers of max_batch_cells rows (device) plus pinned host staging.
+//
+// A launch packs its requests' rows into the slot buffers (device-to-device), integrates them
+// in place with the persistent BDF kernel, scatters them back to their streams and stages the
+// rows callers asked for in pinned memory; an event marks the end. The scheduler thread polls
+// the slots' events, completes finished batches (copies staged rows to caller buffers, wakes
+// waiters) and fills free slots from the pending queue in arrival order.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+using namespace hpo::capi;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct SrvStream {
+  int64_t id = 0;
+  int mech = 0;
+  int64_t n = 0, ld = 0, bytes = 0;
+  double *d_y = nullptr, *d_P = nullptr;
+  int32_t* d_status = nullptr;
+  bool in_flight = false;   // a substep of this stream is pending or running
+};
+
+struct SrvReq {
+  int64_t id = 0;
+  SrvStream* st = nullptr;
+  double dt = 0;
+  double* y_out = nullptr;
+  int64_t ld_out = 0;
+  int32_t* status_out = nullptr;
+  Clock::time_point t_submit, t_launch, t_done;
+  int64_t offset = 0;          // first row in its batch
+  int64_t batch_cells = 0;
+  int32_t batch_requests = 0;
+  int32_t failed = 0;
+  bool done = false;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  ExecRes res;
+  int64_t cap = 0, ldmax = 0;
+  double *d_y = nullptr, *d_P = nullptr, *d_dt = nullptr;
+  int32_t* d_status = nullptr;
+  double *h_y = nullptr, *h_dt = nullptr;   // pinned
+  int32_t* h_status = nullptr;              // pinned
+  std::vector<SrvReq*> batch;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct hpo_server {
+  std::vector<hpo_mech*> mechs;
+  hpo_server_opts opts{};
+  int device = 0;
+  std::vector<Slot> slots;
+  std::map<int64_t, std::unique_ptr<SrvStream>> streams;
+  std::map<int64_t, std::unique_ptr<SrvReq>> reqs;
+  std::deque<SrvReq*> pending;
+  int64_t next_stream = 1, next_req = 1;
+  int64_t used = 0, budget = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  std::thread th;
+  std::string thread_error;
+
+  void release() {
+    for (auto& sl : slots) {
+      for (void* p : {static_cast<void*>(sl.d_y), static_cast<void*>(sl.d_P), static_cast<void*>(sl.d_dt),
+                      static_cast<void*>(sl.d_status), static_cast<void*>(sl.res.d_queue),
+                      static_cast<void*>(sl.res.d_scratch)})
+        if (p) (void)hipFree(p);
+      for (void* p : {static_cast<void*>(sl.h_y), static_cast<void*>(sl.h_dt), static_cast<void*>(sl.h_status)})
+        if (p) (void)hipHostFree(p);
+      if (sl.done) (void)hipEventDestroy(sl.done);
+      if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    slots.clear();
+    for (auto& kv : streams) free_stream(*kv.second);
+    streams.clear();
+  }
+
+  static void free_stream(SrvStream& s) {
+    for (void* p : {static_cast<void*>(s.d_y), static_cast<void*>(s.d_P), static_cast<void*>(s.d_status)})
+      if (p) (void)hipFree(p);
+    s.d_y = s.d_P = nullptr;
+    s.d_status = nullptr;
+  }
+
+  // Packs a batch into slot k and launches it (called with the lock held).
+  void launch(int k, std::vector<SrvReq*>&& batch) {
+    Slot& sl = slots[k];
+    hpo_mech* m = mechs[batch.front()->st->mech];
+    const int64_t ld = m->blob.lay.K;
+    int64_t total = 0;
+    for (SrvReq* r : batch) total += r->st->n;
+    const auto now = Clock::now();
+    int64_t off = 0;
+    for (SrvReq* r : batch) {
+      SrvStream& st = *r->st;
+      r->offset = off;
+      r->t_launch = now;
+      r->batch_cells = total;
+      r->batch_requests = static_cast<int32_t>(batch.size());
+      HPO_HIP(hipMemcpyAsync(sl.d_y + off * ld, st.d_y, st.n * ld * sizeof(double), hipMemcpyDeviceToDevice,
+                             sl.stream));
+      HPO_HIP(hipMemcpyAsync(sl.d_P + off, st.d_P, st.n * sizeof(double), hipMemcpyDeviceToDevice, sl.stream));
+      for (int64_t i = 0; i < st.n; ++i) sl.h_dt[off + i] = r->dt;
+      off += st.n;
+    }
+    HPO_HIP(hipMemcpyAsync(sl.d_dt, sl.h_dt, total * sizeof(double), hipMemcpyHostToDevice, sl.stream));
+    launch_integrate(m, total, sl.d_y, sl.d_y, ld, sl.d_P, sl.d_dt, sl.d_status, nullptr, nullptr, nullptr,
+                     opts.opts, sl.stream, nullptr, &sl.res);
+    for (SrvReq* r : batch) {
+      SrvStream& st = *r->st;
+      HPO_HIP(hipMemcpyAsync(st.d_y, sl.d_y + r->offset * ld, st.n * ld * sizeof(double), hipMemcpyDeviceToDevice,
+                             sl.stream));
+      HPO_HIP(hipMemcpyAsync(st.d_status, sl.d_status + r->offset, st.n * sizeof(int32_t),
+                             hipMemcpyDeviceToDevice, sl.stream));
+      if (r->y_out)
+        HPO_HIP(hipMemcpyAsync(sl.h_y + r->offset * ld, sl.d_y + r->offset * ld, st.n * ld * sizeof(double),
+                               hipMemcpyDeviceToHost, sl.stream));
+    }
+    HPO_HIP(hipMemcpyAsync(sl.h_status, sl.d_status, total * sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
+    HPO_HIP(hipEventRecord(sl.done, sl.stream));
+    sl.batch = std::move(batch);
+    sl.busy = true;
+  }
+
+  // Completes the finished batch of slot k (lock held).
+  void complete(int k) {
+    Slot& sl = slots[k];
+    const auto now = Clock::now();
+    for (SrvReq* r : sl.batch) {
+      SrvStream& st = *r->st;
+      const int64_t ld = st.ld;
+      if (r->y_out)
+        for (int64_t i = 0; i < st.n; ++i)
+          std::memcpy(r->y_out + i * r->ld_out, sl.h_y + (r->offset + i) * ld, ld * sizeof(double));
+      int32_t failed = 0;
+      for (int64_t i = 0; i < st.n; ++i) failed += (sl.h_status[r->offset + i] != HPO_CELL_SUCCESS);
+      if (r->status_out) std::memcpy(r->status_out, sl.h_status + r->offset, st.n * sizeof(int32_t));
+      r->failed = failed;
+      r->t_done = now;
+      r->done = true;
+      st.in_flight = false;
+    }
+    sl.batch.clear();
+    sl.busy = false;
+    cv_done.notify_all();
+  }
+
+  // Next batch in arrival order: the oldest pending request fixes the mechanism; later
+  // requests of that mechanism join while the batch has room (lock held).
+  std::vector<SrvReq*> take_batch(int64_t cap) {
+    std::vector<SrvReq*> b;
+    int mech = -1;
+    int64_t cells = 0;
+    for (auto it = pending.begin(); it != pending.end();) {
+      SrvReq* r = *it;
+      if ((mech < 0 || r->st->mech == mech) && cells + r->st->n <= cap) {
+        mech = r->st->mech;
+        cells += r->st->n;
+        b.push_back(r);
+        it = pending.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    return b;
+  }
+
+  void run() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    try {
+      for (;;) {
+        bool any_busy = false;
+        for (int k = 0; k < static_cast<int>(slots.size()); ++k) {
+          if (!slots[k].busy) continue;
+          hipError_t q = hipEventQuery(slots[k].done);
+          if (q == hipSuccess)
+            complete(k);
+          else if (q == hipErrorNotReady)
+            any_busy = true;
+          else
+            HPO_HIP(q);
+        }
+        if (stop && !any_busy) break;
+        bool launched = false;
+        if (!stop)
+          for (int k = 0; k < static_cast<int>(slots.size()) && !pending.empty(); ++k) {
+            if (slots[k].busy) continue;
+            std::vector<SrvReq*> b = take_batch(slots[k].cap);
+            if (b.empty()) break;
+            launch(k, std::move(b));
+            launched = true;
+            any_busy = true;
+          }
+        if (launched) continue;
+        if (any_busy) {
+          // poll the running batches without holding the lock
+          lk.unlock();
+          std::this_thread::sleep_for(std::chrono::microseconds(50));
+          lk.lock();
+        } else {
+          cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+        }
+      }
+    } catch (const std::exception& e) {
+      thread_error = e.what();
+      stop = true;
+      for (auto& kv : reqs) kv.second->done = true;
+      cv_done.notify_all();
+    }
+  }
+};
+
+extern "C" {
+
+void hpo_server_opts_default(hpo_server_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->n_slots = 4;
+  o->max_batch_cells = 262144;
+  hpo_opts_default(&o->opts);
+}
+
+int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_opts* o, hpo_server** out) {
+  if (!mechs || n_mechs <= 0 || !out) return fail(HPO_E_INVALID, "bad argument");
+  *out = nullptr;
+  auto s = std::make_unique<hpo_server>();
+  try {
+    hpo_server_opts_default(&s->opts);
+    if (o) s->opts = *o;
+    if (s->opts.n_slots <= 0) s->opts.n_slots = 4;
+    if (s->opts.n_slots > 16) throw std::invalid_argument("n_slots must be at most 16");
+    if (s->opts.max_batch_cells <= 0) s->opts.max_batch_cells = 262144;
+    check_opts(s->opts.opts);
+    if (s->opts.opts.sort_cells == 2) throw std::invalid_argument("sort_cells = 2 needs a batch");
+    HPO_HIP(hipGetDevice(&s->device));
+    int64_t stride = 0, wslots = 0, ldmax = 0;
+    for (int i = 0; i < n_mechs; ++i) {
+      if (!mechs[i]) throw std::invalid_argument("null mechanism");
+      hpo_mech* m = mechs[i];
+      if (!hpo::kernel_supports(m->blob.lay.K)) throw std::invalid_argument("no kernel for a mechanism's state size");
+      m->ensure_device();
+      s->mechs.push_back(m);
+      stride = std::max<int64_t>(stride, m->scratch_stride);
+      wslots = std::max<int64_t>(wslots, max_wave_slots(m));
+      ldmax = std::max<int64_t>(ldmax, m->blob.lay.K);
+    }
+    size_t fr = 0, to = 0;
+    HPO_HIP(hipMemGetInfo(&fr, &to));
+    s->slots.resize(s->opts.n_slots);
+    for (Slot& sl : s->slots) {
+      sl.cap = s->opts.max_batch_cells;
+      sl.ldmax = ldmax;
+      HPO_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+      HPO_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+      sl.res.d_queue = dalloc<uint32_t>(4);
+      sl.res.scratch_stride = stride;
+      sl.res.scratch_slots = wslots;
+      sl.res.d_scratch = dalloc<double>(static_cast<size_t>(stride * wslots));
+      sl.d_y = dalloc<double>(static_cast<size_t>(sl.cap * ldmax));
+      sl.d_P = dalloc<double>(sl.cap);
+      sl.d_dt = dalloc<double>(sl.cap);
+      sl.d_status = dalloc<int32_t>(sl.cap);
+      HPO_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_y), sl.cap * ldmax * sizeof(double), hipHostMallocDefault));
+      HPO_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_dt), sl.cap * sizeof(double), hipHostMallocDefault));
+      HPO_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_status), sl.cap * sizeof(int32_t), hipHostMallocDefault));
+    }
+    s->budget = s->opts.hbm_budget_bytes > 0 ? s->opts.hbm_budget_bytes : static_cast<int64_t>(0.9 * fr);
+    s->th = std::thread([p = s.get()] { p->run(); });
+    *out = s.release();
+    return HPO_OK;
+  } catch (...) {
+    s->release();
+    return map_exceptions();
+  }
+}
+
+void hpo_server_destroy(hpo_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  (void)hipSetDevice(s->device);
+  s->release();
+  delete s;
+}
+
+int hpo_server_open_stream(hpo_server* s, int32_t mech, int64_t n, const double* y0, int64_t ld, const double* P,
+                           double P_scalar, int64_t* id) {
+  if (!s || !y0 || !id || n <= 0) return fail(HPO_E_INVALID, "bad argument");
+  if (mech < 0 || mech >= static_cast<int32_t>(s->mechs.size())) return fail(HPO_E_INVALID, "no such mechanism");
+  const int64_t K = s->mechs[mech]->blob.lay.K;
+  if (ld < K) return fail(HPO_E_INVALID, "ld smaller than the state length");
+  if (n > s->opts.max_batch_cells) return fail(HPO_E_INVALID, "stream larger than a launch (max_batch_cells)");
+  const int64_t bytes = n * (K * 8 + 8 + 4);
+  auto st = std::make_unique<SrvStream>();
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->thread_error.empty()) return fail(HPO_E_DEVICE, s->thread_error);
+    if (s->used + bytes > s->budget) return fail(HPO_E_BUSY, "HBM budget of the server exhausted");
+    s->used += bytes;
+    st->id = s->next_stream++;
+  }
+  try {
+    HPO_HIP(hipSetDevice(s->device));
+    st->mech = mech;
+    st->n = n;
+    st->ld = K;
+    st->bytes = bytes;
+    st->d_y = dalloc<double>(static_cast<size_t>(n * K));
+    st->d_P = dalloc<double>(n);
+    st->d_status = dalloc<int32_t>(n);
+    HPO_HIP(hipMemcpy2D(st->d_y, K * sizeof(double), y0, ld * sizeof(double), K * sizeof(double), n,
+                        hipMemcpyHostToDevice));
+    std::vector<double> Pv;
+    if (!P) Pv.assign(n, P_scalar);
+    HPO_HIP(hipMemcpy(st->d_P, P ? P : Pv.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  } catch (...) {
+    hpo_server::free_stream(*st);
+    std::lock_guard<std::mutex> g(s->mu);
+    s->used -= bytes;
+    return map_exceptions();
+  }
+  std::lock_guard<std::mutex> g(s->mu);
+  *id = st->id;
+  s->streams[st->id] = std::move(st);
+  return HPO_OK;
+}
+
+int hpo_server_close_stream(hpo_server* s, int64_t id) {
+  if (!s) return fail(HPO_E_INVALID, "null server");
+  std::unique_ptr<SrvStream> st;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->streams.find(id);
+    if (it == s->streams.end()) return fail(HPO_E_INVALID, "no such stream");
+    if (it->second->in_flight) return fail(HPO_E_BUSY, "a substep of this stream is in flight");
+    st = std::move(it->second);
+    s->streams.erase(it);
+    s->used -= st->bytes;
+  }
+  (void)hipSetDevice(s->device);
+  hpo_server::free_stream(*st);
+  return HPO_OK;
+}
+
+int hpo_server_submit(hpo_server* s, int64_t sid, double dt, double* y_out, int64_t ld_out, int32_t* status_out,
+                      int64_t* rid) {
+  if (!s || !rid) return fail(HPO_E_INVALID, "bad argument");
+  if (!(dt > 0.0)) return fail(HPO_E_INVALID, "dt must be positive");
+  std::lock_guard<std::mutex> g(s->mu);
+  if (!s->thread_error.empty()) return fail(HPO_E_DEVICE, s->thread_error);
+  auto it = s->streams.find(sid);
+  if (it == s->streams.end()) return fail(HPO_E_INVALID, "no such stream");
+  SrvStream& st = *it->second;
+  if (y_out && ld_out < st.ld) return fail(HPO_E_INVALID, "ld_out smaller than the state length");
+  if (st.in_flight) return fail(HPO_E_BUSY, "the previous substep of this stream has not completed");
+  auto r = std::make_unique<SrvReq>();
+  r->id = s->next_req++;
+  r->st = &st;
+  r->dt = dt;
+  r->y_out = y_out;
+  r->ld_out = ld_out;
+  r->status_out = status_out;
+  r->t_submit = Clock::now();
+  st.in_flight = true;
+  *rid = r->id;
+  s->pending.push_back(r.get());
+  s->reqs[r->id] = std::move(r);
+  s->cv_work.notify_one();
+  return HPO_OK;
+}
+
+int hpo_server_wait(hpo_server* s, int64_t rid, double timeout_s, hpo_request_info* info) {
+  if (!s) return fail(HPO_E_INVALID, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(rid);
+  if (it == s->reqs.end()) return fail(HPO_E_INVALID, "no such request (or already waited for)");
+  SrvReq* r = it->second.get();
+  auto ready = [&] { return r->done; };
+  if (timeout_s < 0)
+    s->cv_done.wait(lk, ready);
+  else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
+    return fail(HPO_E_TIMEOUT, "request not completed");
+  if (!s->thread_error.empty()) return fail(HPO_E_DEVICE, s->thread_error);
+  if (info) {
+    info->latency_ms = ms_between(r->t_submit, r->t_done);
+    info->queue_ms = ms_between(r->t_submit, r->t_launch);
+    info->run_ms = ms_between(r->t_launch, r->t_done);
+    info->ncells = r->st->n;
+    info->batch_cells = r->batch_cells;
+    info->batch_requests = r->batch_requests;
+    info->failed_cells = r->failed;
+  }
+  s->reqs.erase(it);
+  return HPO_OK;
+}
+
+int hpo_server_stream_get(hpo_server* s, int64_t sid, double* y, int64_t ld) {
+  if (!s || !y) return fail(HPO_E_INVALID, "bad argument");
+  SrvStream* st = nullptr;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->streams.find(sid);
+    if (it == s->streams.end()) return fail(HPO_E_INVALID, "no such stream");
+    st = it->second.get();
+    if (st->in_flight) return fail(HPO_E_BUSY, "a substep of this stream is in flight");
+  }
+  if (ld < st->ld) return fail(HPO_E_INVALID, "ld smaller than the state length");
+  try {
+    HPO_HIP(hipSetDevice(s->device));
+    HPO_HIP(hipMemcpy2D(y, ld * sizeof(double), st->d_y, st->ld * sizeof(double), st->ld * sizeof(double), st->n,
+                        hipMemcpyDeviceToHost));
+    return HPO_OK;
+  } catch (...) {
+    return map_exceptions();
+  }
+}
+
+int hpo_server_hbm_used(const hpo_server* s, int64_t* used, int64_t* budget) {
+  if (!s) return fail(HPO_E_INVALID, "null server");
+  auto* m = const_cast<hpo_server*>(s);
+  std::lock_guard<std::mutex> g(m->mu);
+  if (used) *used = m->used;
+  if (budget) *budget = m->budget;
+  return HPO_OK;
+}
+
+}  // extern "C"

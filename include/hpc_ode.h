@@ -133,6 +133,56 @@ int hpo_autotune(const hpo_mech *m, int64_t ncells, const double *d_y, int64_t l
 int hpo_batch_autotune(hpo_batch *b, int64_t count, int64_t sample_cells, hpo_opts *o,
                        hpo_tune_report *report, void *hip_stream);
 
+/* ---- serving loop: continuous batching of substep requests over resident cell streams ----
+ * No counterpart in the reference, which integrates one mesh per application run
+ * (app/main.cpp:53-57). For callers that issue chemistry substeps as a CFD solver does:
+ * a stream is a set of cells of one mechanism that stays resident on the device between
+ * substeps (a CFD partition); opening one is admitted against the server's HBM budget.
+ * hpo_server_submit enqueues one substep of a stream. A scheduler thread packs all eligible
+ * pending substeps of one mechanism into one persistent-kernel launch whenever one of the
+ * server's launch slots is free, so requests that arrive while others run join the next
+ * launch, and launches on different slots run on the device at the same time. A stream has at
+ * most one substep in flight; its next substep starts from the state the previous one left.
+ * Each cell is integrated by one wave with the same instruction sequence, so a request's
+ * results do not depend on what it was batched with. */
+typedef struct hpo_server hpo_server;
+typedef struct {
+  int32_t n_slots;           /* launches that may be in flight at once (one HIP stream each); 0 = 4 */
+  int32_t reserved0;
+  int64_t max_batch_cells;   /* cells per launch; 0 = 262144 */
+  int64_t hbm_budget_bytes;  /* device memory for resident streams; 0 = 90% of free HBM at creation */
+  hpo_opts opts;             /* integration options for every launch */
+} hpo_server_opts;
+typedef struct {
+  double latency_ms;         /* submit -> results available to the caller */
+  double queue_ms;           /* submit -> launch of the batch that carried it */
+  double run_ms;             /* launch -> completion of that batch (copies included) */
+  int64_t ncells;
+  int64_t batch_cells;       /* cells in that launch */
+  int32_t batch_requests;    /* requests in that launch */
+  int32_t failed_cells;      /* cells whose solver status is not HPO_CELL_SUCCESS */
+} hpo_request_info;
+#define HPO_E_BUSY (-6)      /* HBM budget exhausted: close a stream and retry */
+#define HPO_E_TIMEOUT (-7)
+void hpo_server_opts_default(hpo_server_opts *o);
+int hpo_server_create(hpo_mech *const *mechs, int32_t n_mechs, const hpo_server_opts *o, hpo_server **out);
+void hpo_server_destroy(hpo_server *s); /* finishes the launches in flight first */
+/* y0: [ncells][ld] host rows [T, Y_0..Y_{K-2}]; P: [ncells] Pa or NULL for P_scalar. */
+int hpo_server_open_stream(hpo_server *s, int32_t mech, int64_t ncells, const double *y0, int64_t ld,
+                           const double *P, double P_scalar, int64_t *stream_id);
+int hpo_server_close_stream(hpo_server *s, int64_t stream_id); /* HPO_E_BUSY while a substep is pending */
+/* One substep of dt seconds. y_out ([ncells][ld_out]) and status_out ([ncells]) receive the
+ * result when the request completes; either may be NULL (state stays on the device only).
+ * The buffers must stay valid until hpo_server_wait returns for this request. */
+int hpo_server_submit(hpo_server *s, int64_t stream_id, double dt, double *y_out, int64_t ld_out,
+                      int32_t *status_out, int64_t *request_id);
+/* Waits for a request (timeout_s < 0: forever); HPO_E_TIMEOUT if it has not completed. A request
+ * can be waited for once. */
+int hpo_server_wait(hpo_server *s, int64_t request_id, double timeout_s, hpo_request_info *info);
+/* Copies a stream's resident state (after its last completed substep) to the host. */
+int hpo_server_stream_get(hpo_server *s, int64_t stream_id, double *y, int64_t ld);
+int hpo_server_hbm_used(const hpo_server *s, int64_t *used_bytes, int64_t *budget_bytes);
+
 /* ---- component entry points (host arrays), used by parity tests ---- */
 /* f = dydt(y) for n states: y, f are [n][n_state]; the pyJac dydt of dydt.c:7-133. */
 int hpo_eval_rhs(const hpo_mech *m, int64_t n, const double *y, const double *P, double *f);
