@@ -49,6 +49,10 @@ def parse():
                          "dt log-U[1e-6,1e-3] s, launch shape and queue order from the autotuner")
     ap.add_argument("--autotune", action="store_true", help="autotune the plain workload too")
     ap.add_argument("--tune-sample", type=int, default=32768)
+    ap.add_argument("--serve-seconds", type=float, default=16.0,
+                    help="duration of the serving-loop measurement, split over the offered loads (0 = skip)")
+    ap.add_argument("--serve-loads", default="0.3,0.7",
+                    help="offered loads as fractions of the measured serving capacity")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -67,6 +71,33 @@ def measured_traffic(workload, ncells, lu_precision):
     if rec.get("workload_kind") != workload or rec.get("cells") != ncells or rec.get("lu_precision") != lu_precision:
         return None
     return rec.get("traffic_bytes")
+
+
+def serving_measurement(H, gri, seconds, fractions):
+    """p50/p99 substep latency of the serving loop at offered loads stated as fractions of its
+    measured capacity (requests: 70 % H2/air, 30 % GRI-3.0, log-uniform [1, 65536] cells,
+    dt U[1e-5, 1e-4] s, two substeps per request stream, Poisson arrivals)."""
+    from hpc_ode_amd import serving as SV
+    h2 = H.Mechanism(os.path.join(ROOT, "hpc_ode_amd", "mechanisms", "h2o2.yaml"))
+    mechs = [h2, gri]
+    srv = H.Server(mechs, n_slots=4)
+    try:
+        cap = SV.capacity(srv, mechs)
+        mc = SV.mean_cells()
+        substeps = 2
+        loads = []
+        for f in fractions:
+            rate = f * cap / (mc * substeps)
+            r = SV.run_load(srv, mechs, rate, seconds / len(fractions), seed=11, substeps=substeps,
+                            log=lambda m: print(m, file=sys.stderr, flush=True))
+            r["fraction_of_capacity"] = f
+            loads.append(r)
+        return {"capacity_cells_per_s": cap, "mean_cells_per_request": mc, "slots": 4,
+                "mix": "70% H2/air (h2o2.yaml), 30% GRI-3.0; cells log-U[1,65536]; dt U[1e-5,1e-4] s; "
+                       "2 substeps per stream with state resident between them; Poisson arrivals",
+                "loads": loads}
+    finally:
+        srv.close()
 
 
 def workload_name(kind, n):
@@ -248,6 +279,10 @@ def main():
         ncheck = min(a.accuracy_cells, n, 1024 if a.workload == "uneven" else n)
         accuracy = accuracy_statement(orc, y0, P, dt, y32, y64, ncheck)
 
+    serving = None
+    if a.serve_seconds > 0:
+        serving = serving_measurement(H, mech, a.serve_seconds, [float(x) for x in a.serve_loads.split(",")])
+
     value = n / (ms32 / 1e3)
     roof = None
     if work is not None:
@@ -272,6 +307,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "accuracy": accuracy,
+        "serving": serving,
         "work_model": {k: counts[k] for k in ("F_rhs", "T_rhs", "F_jac", "F_LU", "F_solve")} if counts else None,
     }
     print(json.dumps(out))
