@@ -649,13 +649,24 @@ HPO_DI float pivot_key(float v, int pos) {
 HPO_DI int key_pos(double k) { return 63 - static_cast<int>(__double_as_longlong(k) & 63); }
 HPO_DI int key_pos(float k) { return 63 - (__float_as_int(k) & 63); }
 
+// Row layout of the factors. FP32: after the elimination every row moves to the lane of its
+// elimination step (one lane permutation per register), so the triangular solves broadcast
+// from fixed lanes; this pays off because the FP32 path solves about twice per Newton
+// iteration (the solve and its refinement). FP64: rows stay in their lanes and the solves
+// look up the pivot lane of each step (the permutation of 106 registers costs more than it
+// saves at one solve per iteration).
+template <typename F>
+constexpr bool kLuPermuted = sizeof(F) == 4;
+
 template <int NS, typename F>
 struct LuReg {
   F a[NS];
-  F rdiag;    // 1/U_kk for the step this lane pivoted
-  int step;   // elimination step at which this lane was the pivot (64: never)
-  int perm;   // lane k: pivot lane of step k
+  F rdiag;    // 1/U_kk: permuted layout, lane k; in place, the lane that pivoted at step k
+  int step;   // in place: elimination step at which this lane was the pivot (64: never)
+  int perm;   // lane k: the lane (original row) that pivoted at step k
 };
+
+HPO_DI float shfl(float v, int src) { return __shfl(v, src, 64); }
 
 // Branch-free over the elimination steps: a zero or NaN pivot column marks the matrix
 // singular and turns the remaining updates into no-ops instead of leaving the loop, so the
@@ -698,28 +709,56 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
       L.a[j] = fma(-l, pj, L.a[j]);
     }
   }
+  if constexpr (kLuPermuted<F>) {
+    const bool live = lane < NS;
+    const int src = live ? L.perm : lane;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) L.a[j] = live ? shfl(L.a[j], src) : F(0);
+    L.rdiag = live ? shfl(L.rdiag, src) : F(0);
+  }
   return !singular;
 }
 
+// Solves (L U) x = P b for the factors of lu_factor; b and x are one entry per lane in the
+// original row / column order.
 template <int NS, typename F>
 HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane) {
+  if constexpr (kLuPermuted<F>) {
+    b = shfl(b, (lane < NS) ? L.perm : lane);
 #pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    int p = rli(L.perm, k);
-    F yk = rl(b, p);
-    F c = (L.step > k) ? L.a[k] : F(0);
-    b = fma(-c, yk, b);
-  }
-  F x = F(0);
+    for (int k = 0; k < NS; ++k) {
+      F yk = rl(b, k);
+      F c = (lane > k) ? L.a[k] : F(0);
+      b = fma(-c, yk, b);
+    }
+    F x = F(0);
 #pragma unroll
-  for (int k = NS - 1; k >= 0; --k) {
-    int p = rli(L.perm, k);
-    F xk = rl(b, p) * rl(L.rdiag, p);
-    x = (lane == k) ? xk : x;
-    F c = (L.step < k) ? L.a[k] : F(0);
-    b = fma(-c, xk, b);
+    for (int k = NS - 1; k >= 0; --k) {
+      F xk = rl(b, k) * rl(L.rdiag, k);
+      x = (lane == k) ? xk : x;
+      F c = (lane < k) ? L.a[k] : F(0);
+      b = fma(-c, xk, b);
+    }
+    return x;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      int p = rli(L.perm, k);
+      F yk = rl(b, p);
+      F c = (L.step > k) ? L.a[k] : F(0);
+      b = fma(-c, yk, b);
+    }
+    F x = F(0);
+#pragma unroll
+    for (int k = NS - 1; k >= 0; --k) {
+      int p = rli(L.perm, k);
+      F xk = rl(b, p) * rl(L.rdiag, p);
+      x = (lane == k) ? xk : x;
+      F c = (L.step < k) ? L.a[k] : F(0);
+      b = fma(-c, xk, b);
+    }
+    return x;
   }
-  return x;
 }
 
 }  // namespace dev
