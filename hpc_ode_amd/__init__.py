@@ -92,6 +92,7 @@ def lib():
         L.hpo_last_error.restype = C.c_char_p
         L.hpo_opts_default.argtypes = [C.POINTER(Opts)]
         L.hpo_mech_load_yaml.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.hpo_mech_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.hpo_mech_destroy.argtypes = [vp]
         L.hpo_mech_info.argtypes = [vp, _ip, _ip, _ip]
         L.hpo_mech_species_name.argtypes = [vp, C.c_int32, C.c_char_p, C.c_int32]
@@ -146,12 +147,12 @@ def opts(rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0, refine_sweeps=1,
 
 
 class Mechanism:
-    """A chemistry mechanism loaded from a Cantera YAML file (the reference's
-    resources/mechanisms/*.yaml), with its tables resident on the GPU."""
+    """A chemistry mechanism loaded from a Cantera YAML or CTML (.xml) file (the reference's
+    resources/mechanisms/), with its tables resident on the GPU."""
 
     def __init__(self, path: str):
         h = C.c_void_p()
-        _check(lib().hpo_mech_load_yaml(path.encode(), C.byref(h)))
+        _check(lib().hpo_mech_load(path.encode(), C.byref(h)))
         self._h = h
         self.path = path
         K, n, R = C.c_int32(), C.c_int32(), C.c_int32()

@@ -760,7 +760,7 @@ static const void* bdf_fn(int lu32) {
               : reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false>);
 }
 
-#define HPO_FOR_EACH_NS(X) X(10) X(53)
+#define HPO_FOR_EACH_NS(X) X(9) X(10) X(53)
 
 int bdf_wave_lds_doubles(const DevMechLayout& L) { return dev::wave_lds_doubles(L.R, L.n_partials) + dev::kScalDoubles; }
 int eval_wave_lds_doubles(const DevMechLayout& L) { return dev::wave_lds_doubles(L.R, L.n_partials); }

@@ -214,12 +214,12 @@ void hpo_opts_default(hpo_opts* o) {
 
 const char* hpo_last_error(void) { return g_err.c_str(); }
 
-int hpo_mech_load_yaml(const char* path, hpo_mech** out) {
+static int mech_load(const char* path, hpo_mech** out, bool by_extension) {
   if (!path || !out) return fail(HPO_E_INVALID, "null argument");
   *out = nullptr;
   try {
     auto m = std::make_unique<hpo_mech>();
-    m->host = hpo::load_mechanism_yaml(path);
+    m->host = by_extension ? hpo::load_mechanism(path) : hpo::load_mechanism_yaml(path);
     m->blob = hpo::build_device_tables(m->host);
     *out = m.release();
     return HPO_OK;
@@ -227,6 +227,10 @@ int hpo_mech_load_yaml(const char* path, hpo_mech** out) {
     return map_exceptions();
   }
 }
+
+int hpo_mech_load_yaml(const char* path, hpo_mech** out) { return mech_load(path, out, false); }
+
+int hpo_mech_load(const char* path, hpo_mech** out) { return mech_load(path, out, true); }
 
 void hpo_mech_destroy(hpo_mech* m) { delete m; }
 

@@ -9,6 +9,7 @@
 #include <sstream>
 #include <stdexcept>
 
+#include "xml_lite.hpp"
 #include "yaml_lite.hpp"
 
 namespace hpo {
@@ -242,6 +243,206 @@ Mechanism load_mechanism_yaml(const std::string& path) {
     m.reactions.push_back(std::move(rx));
   }
   return m;
+}
+
+// ---------------------------------------------------------------------------
+// Cantera CTML (XML) files, e.g. resources/mechanisms/h2_li_2004.xml of the reference.
+// Rate constants are in SI-kmol units already (m, kmol, s); activation energies carry a units
+// attribute. NASA7 thermo, elementary, three-body, Lindemann and Troe falloff reactions.
+// ---------------------------------------------------------------------------
+namespace {
+
+std::vector<std::string> words(const std::string& s) {
+  std::vector<std::string> out;
+  std::istringstream is(s);
+  std::string w;
+  while (is >> w) out.push_back(w);
+  return out;
+}
+
+std::vector<double> numbers(const std::string& s) {
+  std::string t = s;
+  for (char& c : t)
+    if (c == ',') c = ' ';
+  std::vector<double> out;
+  for (const std::string& w : words(t)) out.push_back(std::stod(w));
+  return out;
+}
+
+// "H:2 O:1" -> pairs
+std::vector<std::pair<std::string, double>> name_values(const std::string& s) {
+  std::vector<std::pair<std::string, double>> out;
+  for (const std::string& w : words(s)) {
+    size_t c = w.rfind(':');
+    if (c == std::string::npos) throw std::runtime_error("expected name:value, got '" + w + "'");
+    out.emplace_back(w.substr(0, c), std::stod(w.substr(c + 1)));
+  }
+  return out;
+}
+
+double ea_factor(const std::string& u) {
+  if (u.empty() || u == "cal/mol") return 4184.0;
+  if (u == "kcal/mol") return 4184.0e3;
+  if (u == "J/mol") return 1e3;
+  if (u == "kJ/mol") return 1e6;
+  if (u == "J/kmol") return 1.0;
+  if (u == "K") return kRGas;
+  throw std::runtime_error("activation-energy unit " + u);
+}
+
+void xml_arrhenius(const XNode& a, double& lnA, double& b, double& th, double& A) {
+  const XNode* xa = a.child("A");
+  const XNode* xb = a.child("b");
+  const XNode* xe = a.child("E");
+  if (!xa) throw std::runtime_error("Arrhenius without A");
+  A = std::stod(xa->text);
+  b = xb ? std::stod(xb->text) : 0.0;
+  th = xe ? std::stod(xe->text) * ea_factor(xe->get("units")) / kRGas : 0.0;
+  lnA = A > 0 ? std::log(A) : -INFINITY;
+}
+
+const XNode* find_by_id(const XNode& root, const std::string& tag, const std::string& id) {
+  for (auto& c : root.children) {
+    if (c->name == tag && (id.empty() || c->get("id") == id)) return c.get();
+    if (const XNode* r = find_by_id(*c, tag, id)) return r;
+  }
+  return nullptr;
+}
+
+}  // namespace
+
+Mechanism load_mechanism_xml(const std::string& path) {
+  std::unique_ptr<XNode> doc = xml_load_file(path);
+  Mechanism m;
+  m.path = path;
+  const XNode* phase = find_by_id(*doc, "phase", "");
+  if (!phase) throw std::runtime_error("no <phase> in " + path);
+  const XNode* sa = phase->child("speciesArray");
+  if (!sa) throw std::runtime_error("phase without speciesArray");
+  std::map<std::string, int> index;
+  for (const std::string& n : words(sa->text)) {
+    index[n] = static_cast<int>(m.species.size());
+    m.species.push_back(n);
+  }
+  std::string sd = sa->get("datasrc");
+  const XNode* sdata = find_by_id(*doc, "speciesData", sd.size() > 1 && sd[0] == '#' ? sd.substr(1) : "");
+  if (!sdata) throw std::runtime_error("species data '" + sd + "' not found");
+  const int K = m.K();
+  m.W.assign(K, 0.0);
+  m.tmid.assign(K, 0.0);
+  m.nasa.assign(static_cast<size_t>(K) * 14, 0.0);
+  std::vector<bool> seen(K, false);
+  for (const XNode* sp : sdata->all("species")) {
+    auto it = index.find(sp->get("name"));
+    if (it == index.end()) continue;
+    const int k = it->second;
+    seen[k] = true;
+    double w = 0;
+    if (const XNode* aa = sp->child("atomArray"))
+      for (auto& kv : name_values(aa->text)) w += atomic_weight(kv.first) * kv.second;
+    m.W[k] = w;
+    const XNode* th = sp->child("thermo");
+    if (!th) throw std::runtime_error(m.species[k] + ": no thermo");
+    std::vector<const XNode*> blocks = th->all("NASA");
+    if (blocks.empty()) throw std::runtime_error(m.species[k] + ": only NASA7 thermo is supported");
+    std::sort(blocks.begin(), blocks.end(),
+              [](const XNode* a, const XNode* b) { return std::stod(a->get("Tmin")) < std::stod(b->get("Tmin")); });
+    const XNode* lo = blocks.front();
+    const XNode* hi = blocks.back();
+    auto coeffs = [&](const XNode* b) {
+      const XNode* fa = b->child("floatArray");
+      if (!fa) throw std::runtime_error(m.species[k] + ": NASA block without coefficients");
+      std::vector<double> c = numbers(fa->text);
+      if (c.size() != 7) throw std::runtime_error(m.species[k] + ": NASA block needs 7 coefficients");
+      return c;
+    };
+    std::vector<double> cl = coeffs(lo), ch = coeffs(hi);
+    m.tmid[k] = std::stod(lo->get("Tmax"));
+    for (int j = 0; j < 7; ++j) {
+      m.nasa[14 * k + j] = cl[j];
+      m.nasa[14 * k + 7 + j] = ch[j];
+    }
+  }
+  for (int k = 0; k < K; ++k)
+    if (!seen[k]) throw std::runtime_error("species '" + m.species[k] + "' has no entry");
+
+  const XNode* ra = phase->child("reactionArray");
+  if (!ra) return m;
+  std::string rd = ra->get("datasrc");
+  const XNode* rdata = find_by_id(*doc, "reactionData", rd.size() > 1 && rd[0] == '#' ? rd.substr(1) : "");
+  if (!rdata) throw std::runtime_error("reaction data '" + rd + "' not found");
+  for (const XNode* x : rdata->all("reaction")) {
+    Reaction rx;
+    const XNode* eqn = x->child("equation");
+    rx.equation = eqn ? eqn->text : x->get("id");
+    rx.reversible = x->get("reversible", "yes") == "yes";
+    if (x->child("order")) throw std::runtime_error("non-standard reaction orders unsupported: " + rx.equation);
+    std::vector<std::pair<std::string, double>> rt, pt;
+    if (const XNode* r = x->child("reactants")) rt = name_values(r->text);
+    if (const XNode* p = x->child("products")) pt = name_values(p->text);
+    rx.reactants = merge(rt, index);
+    rx.products = merge(pt, index);
+    const XNode* rc = x->child("rateCoeff");
+    if (!rc) throw std::runtime_error("reaction without rateCoeff: " + rx.equation);
+    const std::string type = x->get("type", "");
+    const XNode* hiA = nullptr;
+    const XNode* k0 = nullptr;
+    for (const XNode* a : rc->all("Arrhenius")) {
+      if (a->get("name") == "k0") k0 = a;
+      else hiA = a;
+    }
+    if (!hiA) throw std::runtime_error("reaction without Arrhenius rate: " + rx.equation);
+    xml_arrhenius(*hiA, rx.lnA, rx.b, rx.theta, rx.A);
+    std::vector<std::pair<int, double>> effs;
+    double deff = 1.0;
+    if (const XNode* e = rc->child("efficiencies")) {
+      deff = std::stod(e->get("default", "1.0"));
+      for (auto& kv : name_values(e->text)) {
+        auto it = index.find(kv.first);
+        if (it != index.end()) effs.emplace_back(it->second, kv.second);
+      }
+    }
+    std::sort(effs.begin(), effs.end());
+    rx.eff_base = deff;
+    for (auto& kv : effs) rx.eff.emplace_back(kv.first, kv.second - deff);
+    if (type.empty() || type == "elementary") {
+      rx.kind = kElementary;
+    } else if (type == "threeBody") {
+      rx.kind = kThreeBody;
+    } else if (type == "falloff") {
+      if (!k0) throw std::runtime_error("falloff reaction without k0: " + rx.equation);
+      xml_arrhenius(*k0, rx.lnA0, rx.b0, rx.theta0, rx.A0);
+      const XNode* f = rc->child("falloff");
+      const std::string ft = f ? f->get("type", "Lindemann") : "Lindemann";
+      if (ft == "Troe") {
+        std::vector<double> t = numbers(f->text);
+        if (t.size() < 3 || t.size() > 4) throw std::runtime_error("Troe falloff needs 3 or 4 parameters");
+        rx.kind = kTroe;
+        rx.troe[0] = t[0];
+        rx.troe[1] = t[1];
+        rx.troe[2] = t[2];
+        rx.troe_has_t2 = t.size() == 4;
+        rx.troe[3] = rx.troe_has_t2 ? t[3] : 0.0;
+      } else if (ft == "Lindemann") {
+        rx.kind = kLindemann;
+      } else {
+        throw std::runtime_error("falloff type '" + ft + "' unsupported: " + rx.equation);
+      }
+    } else {
+      throw std::runtime_error("reaction type '" + type + "' unsupported: " + rx.equation);
+    }
+    m.reactions.push_back(std::move(rx));
+  }
+  return m;
+}
+
+Mechanism load_mechanism(const std::string& path) {
+  auto ends = [&](const char* e) {
+    const size_t n = std::char_traits<char>::length(e);
+    return path.size() >= n && path.compare(path.size() - n, n, e) == 0;
+  };
+  if (ends(".xml") || ends(".ctml")) return load_mechanism_xml(path);
+  return load_mechanism_yaml(path);
 }
 
 // ---------------------------------------------------------------------------

@@ -47,6 +47,8 @@ struct Mechanism {
 };
 
 Mechanism load_mechanism_yaml(const std::string& path);
+Mechanism load_mechanism_xml(const std::string& path);   // Cantera CTML
+Mechanism load_mechanism(const std::string& path);       // by extension: .xml/.ctml or YAML
 
 // ---------------------------------------------------------------------------
 // Device tables. One contiguous blob; the kernel copies the "LDS part" into

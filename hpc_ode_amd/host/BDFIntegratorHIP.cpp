@@ -69,7 +69,7 @@ void BDFIntegratorHIP::init(std::shared_ptr<Context> c, IntegratorConfig config)
   opts_.atol = abstol;
   try {
     const std::string path = resolveMechanism(mechanism);
-    check(hpo_mech_load_yaml(path.c_str(), &mech_), "loading the mechanism");
+    check(hpo_mech_load(path.c_str(), &mech_), "loading the mechanism");
     int32_t nsp = 0, nstate = 0, nrxn = 0;
     check(hpo_mech_info(mech_, &nsp, &nstate, &nrxn), "mechanism info");
     if (nsp != systemSize) {

@@ -1,4 +1,4 @@
-"""Oracle-side mechanism reader: Cantera YAML -> flat numpy tables.
+"""Oracle-side mechanism reader: Cantera YAML or CTML (XML) -> flat numpy tables.
 
 TEST INFRASTRUCTURE ONLY. This module is part of the CPU checker: only tests,
 the smoke check and the CPU-baseline leg of bench.py may import it. The product
@@ -139,6 +139,99 @@ _Loader.add_implicit_resolver("tag:yaml.org,2002:bool", re.compile(r"^(?:true|fa
 
 
 def load(path: str) -> Mechanism:
+    """Reads a Cantera YAML file, or a CTML file when the path ends in .xml / .ctml."""
+    if path.endswith((".xml", ".ctml")):
+        return load_xml(path)
+    return load_yaml(path)
+
+
+def load_xml(path: str) -> Mechanism:
+    """Cantera CTML: rate constants already in SI-kmol units, activation energies with a units
+    attribute (cal/mol when absent)."""
+    import xml.etree.ElementTree as ET
+    root = ET.parse(path).getroot()
+    phase = root.find(".//phase")
+    names = phase.find("speciesArray").text.split()
+    index = {n: i for i, n in enumerate(names)}
+    sref = phase.find("speciesArray").get("datasrc", "#").lstrip("#")
+    sdata = next(e for e in root.iter("speciesData") if not sref or e.get("id") == sref)
+    K = len(names)
+    W = np.zeros(K)
+    tmid = np.zeros(K)
+    lo = np.zeros((K, 7))
+    hi = np.zeros((K, 7))
+    for sp in sdata.findall("species"):
+        n = sp.get("name")
+        if n not in index:
+            continue
+        k = index[n]
+        W[k] = sum(ATOMIC_WEIGHTS[t.split(":")[0]] * float(t.split(":")[1])
+                   for t in sp.find("atomArray").text.split())
+        blocks = sorted(sp.find("thermo").findall("NASA"), key=lambda b: float(b.get("Tmin")))
+        coef = [np.array([float(v) for v in b.find("floatArray").text.replace(",", " ").split()]) for b in blocks]
+        tmid[k] = float(blocks[0].get("Tmax"))
+        lo[k] = coef[0]
+        hi[k] = coef[-1]
+
+    def ea(e):
+        if e is None:
+            return 0.0
+        u = e.get("units", "cal/mol")
+        return float(e.text) * {"cal/mol": CAL, "kcal/mol": CAL * 1e3, "J/mol": 1e3, "kJ/mol": 1e6,
+                                "J/kmol": 1.0, "K": R_GAS}[u] / R_GAS
+
+    def arr(a):
+        b = a.find("b")
+        return float(a.find("A").text), float(b.text) if b is not None else 0.0, ea(a.find("E"))
+
+    def pairs(e):
+        if e is None or not e.text:
+            return []
+        return [(t.rsplit(":", 1)[0], float(t.rsplit(":", 1)[1])) for t in e.text.split()]
+
+    rref = phase.find("reactionArray").get("datasrc", "#").lstrip("#")
+    rdata = next(e for e in root.iter("reactionData") if not rref or e.get("id") == rref)
+    rxns: List[Reaction] = []
+    for x in rdata.findall("reaction"):
+        eq = x.find("equation").text
+        rev = x.get("reversible", "yes") == "yes"
+        if x.find("order") is not None:
+            raise ValueError("non-standard reaction orders unsupported")
+        reac = _merge(pairs(x.find("reactants")), index)
+        prod = _merge(pairs(x.find("products")), index)
+        rc = x.find("rateCoeff")
+        hiA = [a for a in rc.findall("Arrhenius") if a.get("name") != "k0"][0]
+        A, b, th = arr(hiA)
+        effs, deff = {}, 1.0
+        e = rc.find("efficiencies")
+        if e is not None:
+            deff = float(e.get("default", "1.0"))
+            effs = {index[n]: v for n, v in pairs(e) if n in index}
+        typ = x.get("type", "elementary")
+        if typ == "elementary":
+            rx = Reaction(eq, RXN_ELEMENTARY, rev, reac, prod, A, b, th)
+        elif typ == "threeBody":
+            rx = Reaction(eq, RXN_THREE_BODY, rev, reac, prod, A, b, th, efficiencies=effs, default_eff=deff)
+        elif typ == "falloff":
+            A0, b0, th0 = arr([a for a in rc.findall("Arrhenius") if a.get("name") == "k0"][0])
+            f = rc.find("falloff")
+            ftype = f.get("type", "Lindemann") if f is not None else "Lindemann"
+            if ftype == "Troe":
+                t = [float(v) for v in f.text.split()]
+                kind, troe, has_t2 = RXN_TROE, (t[0], t[1], t[2], t[3] if len(t) > 3 else 0.0), len(t) > 3
+            elif ftype == "Lindemann":
+                kind, troe, has_t2 = RXN_LINDEMANN, (0.0, 0.0, 0.0, 0.0), False
+            else:
+                raise ValueError(f"falloff type {ftype} unsupported")
+            rx = Reaction(eq, kind, rev, reac, prod, A, b, th, A0, b0, th0, efficiencies=effs, default_eff=deff,
+                          troe=troe, troe_has_t2=has_t2)
+        else:
+            raise ValueError(f"reaction type {typ} unsupported")
+        rxns.append(rx)
+    return Mechanism(names, W, tmid, lo, hi, rxns)
+
+
+def load_yaml(path: str) -> Mechanism:
     with open(path) as f:
         doc = yaml.load(f, Loader=_Loader)
     units = doc.get("units", {})

@@ -10,6 +10,7 @@ if ROOT not in sys.path:
 MECH_DIR = os.path.join(ROOT, "hpc_ode_amd", "mechanisms")
 GRI = os.path.join(MECH_DIR, "gri30.yaml")
 H2 = os.path.join(MECH_DIR, "h2o2.yaml")
+H2LI = os.path.join(MECH_DIR, "h2_li_2004.xml")
 
 
 def pytest_configure(config):
@@ -24,3 +25,8 @@ def gri_path():
 @pytest.fixture(scope="session")
 def h2_path():
     return H2
+
+
+@pytest.fixture(scope="session")
+def h2li_path():
+    return H2LI

@@ -26,10 +26,11 @@ def test_no_oracle_in_product_library():
     assert "liboracle" not in ldd and "pyjac" not in ldd
 
 
-@pytest.mark.parametrize("which", ["gri", "h2"])
-def test_cpp_reader_matches_oracle_reader(which, gri_path, h2_path):
-    """The product's C++ YAML reader and the oracle's PyYAML reader give identical tables."""
-    path = gri_path if which == "gri" else h2_path
+@pytest.mark.parametrize("which", ["gri", "h2", "h2li"])
+def test_cpp_reader_matches_oracle_reader(which, gri_path, h2_path, h2li_path):
+    """The product's C++ readers (YAML; CTML for h2_li_2004.xml) and the oracle's readers
+    (PyYAML; xml.etree) give identical tables."""
+    path = {"gri": gri_path, "h2": h2_path, "h2li": h2li_path}[which]
     m = H.Mechanism(path)
     o = O.Oracle(path)
     assert m.species == o.mech.species

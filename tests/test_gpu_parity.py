@@ -186,3 +186,18 @@ def test_accuracy_vs_tight_reference(gri, lu_precision):
     assert eT_g < 5e-4 and eY_g < 2e-2
     assert eT_g <= 3 * eT_o + 1e-7
     assert eY_g <= 3 * eY_o + 1e-5
+
+
+def test_rhs_and_jacobian_h2li_xml_mechanism(h2li_path):
+    """The 9-species CTML mechanism (h2_li_2004.xml) through the component kernels."""
+    import oracle as O
+    m, o = H.Mechanism(h2li_path), O.Oracle(h2li_path)
+    y, P, _ = WL.h2_air(m.species, m.W, 64, seed=41)
+    rng = np.random.default_rng(42)
+    for s in ("H", "O", "OH", "HO2"):
+        y[:32, 1 + m.species.index(s)] = rng.uniform(1e-6, 1e-3, 32)
+    _rhs_check(m, o, y, P, 1e-10)
+    J = m.jac(y, P)
+    ref = np.stack([o.jac(P[i], y[i]) for i in range(len(y))])
+    scale = np.abs(ref).max(axis=(1, 2), keepdims=True)
+    assert (np.abs(J - ref) / scale).max() < 1e-10

@@ -146,3 +146,32 @@ def test_h2_cells_through_csv_path_match_oracle(tmp_path, h2_path):
     assert w.max() < 100.0
     # last species closes the mass balance, as CVodeIntegrator.cpp:90-96
     assert np.allclose(state[:, 1:].sum(axis=1), 1.0, rtol=0, atol=1e-13)
+
+
+@pytest.mark.gpu
+def test_reference_h2_states_with_xml_mechanism(tmp_path, h2li_path):
+    """500 of the 10,000 hot H2/air states of the reference's ref_data/res_h2.csv (every 20th
+    row), with their CTML mechanism h2_li_2004.xml (ref_data/README.md:15), read by the CSV
+    reader, integrated for 1e-4 s at 101325 Pa through BDFIntegratorHIP and compared with the
+    oracle in full precision."""
+    import oracle as O
+    src = os.path.join(GOLD, "res_h2_sample.csv")
+    raw = tmp_path / "h2li.bin"
+    r = subprocess.run([_exe("hpo_integrate_csv"), "--input", src, "--output", str(tmp_path / "o.csv"),
+                        "--mechanism", h2li_path, "--dt", "1e-4", "--pressure", "101325", "--raw", str(raw),
+                        "--quiet"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, d = _read_csv(src)
+    n, K = d.shape[0], 9
+    state, status = _read_raw(str(raw), n, K)
+    assert (status == 0).all()
+    y0 = np.concatenate([d[:, -1:], d[:, :K - 1]], axis=1)
+    orc = O.Oracle(h2li_path)
+    yo, so, _ = orc.integrate(np.full(n, 101325.0), np.full(n, 1e-4), y0, nthreads=8)
+    assert (so == 0).all()
+    yg = state[:, :K]
+    assert np.max(np.abs(yg[:, 0] - yo[:, 0]) / yo[:, 0]) < 1e-5
+    assert np.max(np.abs(yg - yo) / (1e-6 * np.abs(yo) + 1e-10)) < 100.0
+    # enthalpy passes through unchanged, and the written file keeps the input's columns
+    head, out = _read_csv(str(tmp_path / "o.csv"))
+    assert head[K:] == ["ENTHA", "TEMPE"] and np.allclose(out[:, K], d[:, K], rtol=1e-6)

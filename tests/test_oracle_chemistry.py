@@ -193,3 +193,23 @@ def test_equilibrium_has_zero_temperature_rate(gri):
     f = gri.rhs(P[0], ye[0])
     f0 = gri.rhs(P[0], y0)
     assert abs(f[0]) < 1e-6 * abs(f0[0])
+
+
+def test_h2li_xml_mechanism_jacobian_and_conservation(h2li_path):
+    """The CTML mechanism of the reference's res_h2.csv (h2_li_2004.xml, 9 species, 21
+    reactions): the analytic Jacobian matches central differences on the reference's own
+    states, and production rates conserve mass."""
+    o = O.Oracle(h2li_path)
+    d = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "res_h2_sample.csv"), delimiter=",",
+                   skiprows=1)
+    tmid = set(o.mech.tmid.tolist())
+    for row in d[::50]:
+        y = np.concatenate([[row[-1]], row[:8]])          # [T, Y_0..Y_7]; N2 (last) eliminated
+        if min(abs(y[0] - t) for t in tmid) < 1.0:
+            continue   # central differences would straddle the NASA7 range switch
+        yy = y.copy()
+        for s in ("H", "O", "OH", "HO2"):
+            yy[1 + o.mech.species.index(s)] = max(yy[1 + o.mech.species.index(s)], 1e-4)
+        assert _check_fd(o, 101325.0, yy, 1e-5) < 1e-5
+        w = o.rates(101325.0, y)["wdot"]
+        assert abs((w * o.mech.W).sum()) <= 1e-11 * np.abs(w * o.mech.W).max() + 1e-300
