@@ -667,10 +667,19 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
   wave_sync();
 }
 
+// Small mechanisms (at most 16 state variables: a 16-register Newton row, one or two reaction
+// passes) fit several waves per SIMD, which hides the latency one wave exposes.
+#ifndef HPO_WAVES_PER_EU_SMALL
+#define HPO_WAVES_PER_EU_SMALL 2
+#endif
+template <int NS, bool LU32>
+constexpr int bdf_waves_per_eu() {
+  return NS <= 16 ? HPO_WAVES_PER_EU_SMALL : (LU32 ? HPO_WAVES_PER_EU_FP32 : HPO_WAVES_PER_EU_FP64);
+}
+
 template <int NS, bool LU32>
 __global__ __launch_bounds__(256) __attribute__((
-    amdgpu_waves_per_eu(LU32 ? HPO_WAVES_PER_EU_FP32 : HPO_WAVES_PER_EU_FP64,
-                        LU32 ? HPO_WAVES_PER_EU_FP32 : HPO_WAVES_PER_EU_FP64))) void bdf_kernel(BdfArgs A) {
+    amdgpu_waves_per_eu(bdf_waves_per_eu<NS, LU32>(), bdf_waves_per_eu<NS, LU32>()))) void bdf_kernel(BdfArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
   const DevMechLayout& Lay = A.lay;
   for (int i = threadIdx.x * 16; i < Lay.lds_bytes; i += blockDim.x * 16)
