@@ -4,12 +4,14 @@
 //
 //   hpo_integrate_csv --input cells.csv --output result.csv [--mechanism gri30.yaml]
 //       [--dt 1e-3] [--pressure 101325.15] [--rtol 1e-6] [--atol 1e-10] [--lu fp64|fp32]
-//       [--sweeps 1] [--raw state.bin] [--quiet]
+//       [--sweeps 1] [--raw state.bin] [--performance-yaml performance.yaml] [--quiet]
 //
 // --raw writes the integrated mesh in full precision: per cell, T then the nsp mass
 // fractions, as little-endian doubles, followed by the int32 solver status of every cell.
 // Prints one JSON line: cells, species, integrate seconds (wall, as the reference's
-// integrateTimer), kernel milliseconds, failed cells.
+// integrateTimer), kernel milliseconds, failed cells. --performance-yaml writes the reference
+// application's performance.yaml (readTime, integrationTime, writeTime, totalTime in seconds,
+// app/main.cpp:65-85), which its Python runner reads (app/main.py:157-168).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -22,7 +24,7 @@
 #include "csvIO.hpp"
 
 int main(int argc, char** argv) {
-  std::string in, out, raw, mech = "gri30.yaml", lu = "fp64";
+  std::string in, out, raw, perf, mech = "gri30.yaml", lu = "fp64";
   double dt = 1e-3, P = 101325.15, rtol = 1e-6, atol = 1e-10;
   int sweeps = 1;
   bool quiet = false;
@@ -42,6 +44,7 @@ int main(int argc, char** argv) {
     else if (a == "--lu") lu = val();
     else if (a == "--sweeps") sweeps = std::stoi(val());
     else if (a == "--raw") raw = val();
+    else if (a == "--performance-yaml") perf = val();
     else if (a == "--quiet") quiet = true;
     else {
       std::fprintf(stderr, "unknown argument %s\n", a.c_str());
@@ -56,7 +59,9 @@ int main(int argc, char** argv) {
     auto logger = quiet ? std::shared_ptr<BaseLogger>(std::make_shared<ConsoleLogger>(LogLevel::ERROR))
                         : std::shared_ptr<BaseLogger>(std::make_shared<ConsoleLogger>(LogLevel::INFO));
     auto ctx = std::make_shared<Context>(logger, std::make_shared<Mesh>());
+    auto r0 = std::chrono::steady_clock::now();
     csvReader(ctx, in).read();
+    auto r1 = std::chrono::steady_clock::now();
 
     hpo_opts o;
     hpo_opts_default(&o);
@@ -72,12 +77,24 @@ int main(int argc, char** argv) {
     BDFIntegratorHIP integ(&o);
     integ.init(ctx, cfg);
     integ.integrate(0.0, dt);
-    auto t1 = std::chrono::steady_clock::now();
     std::vector<int32_t> status = integ.cellStatus();
     const double kernel_ms = integ.lastTiming().kernel_ms;
     integ.clean();
+    auto t1 = std::chrono::steady_clock::now();   // init + integrate + clean, as integrateTimer
 
+    auto w0 = std::chrono::steady_clock::now();
     csvWriter(ctx, out).write();
+    auto w1 = std::chrono::steady_clock::now();
+    if (!perf.empty()) {
+      const double tr = std::chrono::duration<double>(r1 - r0).count();
+      const double ti = std::chrono::duration<double>(t1 - t0).count();
+      const double tw = std::chrono::duration<double>(w1 - w0).count();
+      std::FILE* f = std::fopen(perf.c_str(), "w");
+      if (!f) throw std::runtime_error("cannot open " + perf);
+      std::fprintf(f, "readTime: %.17g\nintegrationTime: %.17g\nwriteTime: %.17g\ntotalTime: %.17g\n", tr, ti, tw,
+                   tr + ti + tw);
+      std::fclose(f);
+    }
     Mesh& m = *ctx->getMesh();
     if (!raw.empty()) {
       std::FILE* f = std::fopen(raw.c_str(), "wb");

@@ -89,12 +89,17 @@ def test_reference_e2e_100_cells(tmp_path, lu, gri_path):
     at dt = 1e-3 s, 101325.15 Pa (app/config.yaml:16-19). Its file holds 5 significant digits:
     T and every mass fraction above 1e-10 must agree to that precision (the cells are 300 K
     mixtures, so the printed state is unchanged by the step)."""
-    out, raw = tmp_path / "result.csv", tmp_path / "state.bin"
+    out, raw, perf = tmp_path / "result.csv", tmp_path / "state.bin", tmp_path / "performance.yaml"
     r = subprocess.run([_exe("hpo_integrate_csv"), "--input", os.path.join(GOLD, "res_gri_100.csv"),
                         "--output", str(out), "--mechanism", gri_path, "--dt", "1e-3",
-                        "--pressure", "101325.15", "--lu", lu, "--raw", str(raw), "--quiet"],
-                       capture_output=True, text=True, timeout=300)
+                        "--pressure", "101325.15", "--lu", lu, "--raw", str(raw), "--quiet",
+                        "--performance-yaml", str(perf)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
+    # the reference application's performance.yaml keys (app/main.cpp:76-79), read as its runner does
+    import yaml
+    p = yaml.safe_load(perf.read_text())
+    assert set(p) == {"readTime", "integrationTime", "writeTime", "totalTime"}
+    assert abs(p["totalTime"] - (p["readTime"] + p["integrationTime"] + p["writeTime"])) < 1e-9
     _, ref = _read_csv(os.path.join(GOLD, "results_serial100.csv"))
     n = ref.shape[0]
     state, status = _read_raw(str(raw), n, 53)
