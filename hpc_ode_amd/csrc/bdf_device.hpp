@@ -124,6 +124,7 @@ HPO_DI void put(T (&v)[6], int i, T x) {
 struct Tabs {
   const double* sp;        // [K][kSpStride]
   const RxnHdr* hdr;       // [R]
+  const RxnMol* mol;       // [R]
   const double* arr;       // [R][3]
   const PdepRec* pd;       // [n_pdep]
   const double* effv;      // [n_eff]
@@ -135,13 +136,15 @@ struct Tabs {
   const uint16_t* pptr;    // [K+1]
   const uint16_t* slot_base;  // [R+1] (global)
   const uint64_t* jterm;      // [njl][64] (global)
-  int K, R, ngl, njl, npart, neffp, npdep;
+  int K, R, ngl, njl, npart, neffp, npdep, mol_ok;
 };
 
 HPO_DI Tabs make_tabs(const uint8_t* lds, const uint8_t* glob, const DevMechLayout& L) {
   Tabs t;
   t.sp = reinterpret_cast<const double*>(lds + L.off_species);
   t.hdr = reinterpret_cast<const RxnHdr*>(lds + L.off_hdr);
+  t.mol = reinterpret_cast<const RxnMol*>(lds + L.off_mol);
+  t.mol_ok = L.mol_ok;
   t.arr = reinterpret_cast<const double*>(lds + L.off_arr);
   t.pd = reinterpret_cast<const PdepRec*>(lds + L.off_pdep);
   t.effv = reinterpret_cast<const double*>(lds + L.off_eff_val);
@@ -296,25 +299,34 @@ HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part
 }
 
 // Net rate (fwd - rev) * pressure modification of reaction r (rxn_rates.c, rxn_rates_pres_mod.c).
-template <bool PDEP>
+template <bool PDEP, bool MOL>
 HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, int r) {
   const RxnHdr h = t.hdr[r];
   const double* ar = t.arr + 3 * r;
-  // straight-line for the elementary part, so two reactions per lane can be interleaved
+  // straight-line for the elementary part, so several reactions per lane can be interleaved
   double e = exp(ar[0] + ar[1] * s.lnT - ar[2] * s.invT);
   double kf = (h.flags & 2) ? ar[0] : e;
   double fwd = kf, pc = 1.0, dG = 0.0, dnu = 0.0;
+  if constexpr (MOL) {
+    // molecule slots: unused ones read the dummy species (C = 1, g = 0)
+    const RxnMol mo = t.mol[r];
+    fwd *= w.C[mo.m[0]] * w.C[mo.m[1]] * w.C[mo.m[2]];
+    pc = w.C[mo.m[3]] * w.C[mo.m[4]] * w.C[mo.m[5]];
+    dG = (w.g[mo.m[3]] + w.g[mo.m[4]] + w.g[mo.m[5]]) - (w.g[mo.m[0]] + w.g[mo.m[1]] + w.g[mo.m[2]]);
+    dnu = static_cast<double>(mo.dnu);
+  } else {
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    fwd *= cpow_int(w.C[h.sp[q]], h.nu[q]);
-    dG -= h.nu[q] * w.g[h.sp[q]];
-    dnu -= h.nu[q];
-  }
+    for (int q = 0; q < 3; ++q) {
+      fwd *= cpow_int(w.C[h.sp[q]], h.nu[q]);
+      dG -= h.nu[q] * w.g[h.sp[q]];
+      dnu -= h.nu[q];
+    }
 #pragma unroll
-  for (int q = 3; q < 6; ++q) {
-    pc *= cpow_int(w.C[h.sp[q]], h.nu[q]);
-    dG += h.nu[q] * w.g[h.sp[q]];
-    dnu += h.nu[q];
+    for (int q = 3; q < 6; ++q) {
+      pc *= cpow_int(w.C[h.sp[q]], h.nu[q]);
+      dG += h.nu[q] * w.g[h.sp[q]];
+      dnu += h.nu[q];
+    }
   }
   // k_r = k_f / Kc, Kc = (P_atm/(R T))^dnu exp(-dG)
   double rev = (h.flags & 1) ? kf * exp(dG - dnu * s.lnPfac) * pc : 0.0;
@@ -377,20 +389,23 @@ HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane,
   // Pressure-dependent reactions come first in the device order; with at most 64 of them
   // only the first slot of the first pass can hold one, and the other slots take the
   // elementary-only path (a mechanism with more takes the general path in every slot).
-  const bool few_pdep = t.npdep <= 64;
+  // The fast path (molecule slots, at most 64 pressure-dependent reactions) covers the
+  // shipped mechanisms; any other takes the general path in every slot.
+  const bool fast = t.npdep <= 64 && t.mol_ok;
 #pragma unroll 1
   for (int base = 0; base < R; base += 256) {
     const int r0 = base + lane, r1 = r0 + 64, r2 = r0 + 128, r3 = r0 + 192;
-    double p0 = (r0 < R) ? reaction_rate<true>(t, w, s, r0) : 0.0;
-    double p1 = 0.0, p2 = 0.0, p3 = 0.0;
-    if (few_pdep) {
-      p1 = (r1 < R) ? reaction_rate<false>(t, w, s, r1) : 0.0;
-      p2 = (r2 < R) ? reaction_rate<false>(t, w, s, r2) : 0.0;
-      p3 = (r3 < R) ? reaction_rate<false>(t, w, s, r3) : 0.0;
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    if (fast) {
+      p0 = (r0 < R) ? reaction_rate<true, true>(t, w, s, r0) : 0.0;
+      p1 = (r1 < R) ? reaction_rate<false, true>(t, w, s, r1) : 0.0;
+      p2 = (r2 < R) ? reaction_rate<false, true>(t, w, s, r2) : 0.0;
+      p3 = (r3 < R) ? reaction_rate<false, true>(t, w, s, r3) : 0.0;
     } else {
-      p1 = (r1 < R) ? reaction_rate<true>(t, w, s, r1) : 0.0;
-      p2 = (r2 < R) ? reaction_rate<true>(t, w, s, r2) : 0.0;
-      p3 = (r3 < R) ? reaction_rate<true>(t, w, s, r3) : 0.0;
+      p0 = (r0 < R) ? reaction_rate<true, false>(t, w, s, r0) : 0.0;
+      p1 = (r1 < R) ? reaction_rate<true, false>(t, w, s, r1) : 0.0;
+      p2 = (r2 < R) ? reaction_rate<true, false>(t, w, s, r2) : 0.0;
+      p3 = (r3 < R) ? reaction_rate<true, false>(t, w, s, r3) : 0.0;
     }
     if (r0 < R) w.phi[r0] = p0;
     if (r1 < R) w.phi[r1] = p1;

@@ -638,6 +638,8 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   off = align_to(off, 16);
   L.off_hdr = off;
   off += R * static_cast<int>(sizeof(RxnHdr));
+  L.off_mol = off;
+  off += R * static_cast<int>(sizeof(RxnMol));
   L.off_arr = off;
   off += R * 3 * 8;
   L.off_pdep = off;
@@ -675,6 +677,8 @@ DevMechBlob build_device_tables(const Mechanism& m) {
     for (int j = 0; j < 14; ++j) sp[k * kSpStride + 3 + j] = m.nasa[14 * k + j];
   }
   RxnHdr* hdr = reinterpret_cast<RxnHdr*>(B + L.off_hdr);
+  RxnMol* mol = reinterpret_cast<RxnMol*>(B + L.off_mol);
+  L.mol_ok = 1;
   double* arr = reinterpret_cast<double*>(B + L.off_arr);
   PdepRec* pd = reinterpret_cast<PdepRec*>(B + L.off_pdep);
   double* effv = reinterpret_cast<double*>(B + L.off_eff_val);
@@ -692,6 +696,25 @@ DevMechBlob build_device_tables(const Mechanism& m) {
     for (size_t s = 0; s < rx.products.size(); ++s) {
       h.sp[3 + s] = static_cast<uint8_t>(rx.products[s].first);
       h.nu[3 + s] = static_cast<uint8_t>(rx.products[s].second);
+    }
+    {
+      RxnMol mo;
+      std::memset(&mo, 0, sizeof(mo));
+      for (int q = 0; q < 6; ++q) mo.m[q] = kDummySpecies;
+      int nr = 0, np = 0;
+      for (auto& kv : rx.reactants)
+        for (int c = 0; c < static_cast<int>(kv.second); ++c) {
+          if (nr < 3) mo.m[nr] = static_cast<uint8_t>(kv.first);
+          ++nr;
+        }
+      for (auto& kv : rx.products)
+        for (int c = 0; c < static_cast<int>(kv.second); ++c) {
+          if (np < 3) mo.m[3 + np] = static_cast<uint8_t>(kv.first);
+          ++np;
+        }
+      if (nr > 3 || np > 3) L.mol_ok = 0;
+      mo.dnu = static_cast<int8_t>(np - nr);
+      mol[r] = mo;
     }
     h.kind = static_cast<uint8_t>(rx.kind);
     bool constk = (rx.b == 0.0 && rx.theta == 0.0);

@@ -65,6 +65,15 @@ struct alignas(16) RxnHdr {     // 16 bytes
   uint16_t pidx;                // index into pressure-dependent tables (pressure-dependent reactions)
 };
 
+// The same reaction as molecules for the rate evaluation: each side lists up to three
+// molecules (a species with coefficient 2 appears twice), unused slots hold kDummySpecies,
+// so the concentration products and Gibbs sums need no exponents or selects.
+struct alignas(8) RxnMol {      // 8 bytes
+  uint8_t m[6];                 // reactant molecules 0..2, product molecules 3..5
+  int8_t dnu;                   // product molecules - reactant molecules
+  uint8_t pad;
+};
+
 struct alignas(8) PdepRec {     // per pressure-dependent reaction
   double eff_base;
   int32_t eff_begin, eff_end;   // range in eff tables
@@ -81,10 +90,12 @@ struct DevMechLayout {
   int32_t n_jterm_lane;         // Jacobian assembly terms per lane
   int32_t n_coef;               // distinct stoichiometric coefficient values
   int32_t n_eff_pad;            // efficiency entries per pressure-dependent reaction in the padded table
+  int32_t mol_ok;               // 1: every reaction side has at most 3 molecules (RxnMol table valid)
   int32_t lds_bytes;            // bytes [0, lds_bytes) are staged to LDS
   // LDS part
   int32_t off_species;          // double[K][16]: invW, W, tmid, (pad), nasa[14] -> see kSp*
   int32_t off_hdr;              // RxnHdr[R]
+  int32_t off_mol;              // RxnMol[R]
   int32_t off_arr;              // double[R][3]: lnA (or A), b, theta
   int32_t off_pdep;             // PdepRec[n_pdep]
   int32_t off_eff_val;          // double[n_eff]
