@@ -72,7 +72,7 @@ struct Scal {
 static_assert(sizeof(Scal) % 8 == 0, "Scal layout");
 constexpr int kScalDoubles = sizeof(Scal) / 8;
 
-template <int NS, bool LU32>
+template <int NS, bool LU32, bool PROF>
 __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w, const WaveScratch& g, Scal& S,
                                int64_t cell, int lane) {
   const int n = NS;
@@ -111,7 +111,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
   int ncf = 0, nef = 0, nflag = FIRST_CALL, m = 0, need_setup = 0, jbad = 0;
   int status = HPO_SUCCESS;
   int count1 = 0;
-  const bool prof = A.prof != nullptr;
+  constexpr bool prof = PROF;   // per-phase cycle counters: a separate kernel instantiation
   long long c_rhs = 0, c_jac = 0, c_lu = 0, c_sol = 0, c_t0 = prof ? clock64() : 0, c_a = 0;
 
   // error weights for y0 (cvInitialSetup)
@@ -677,7 +677,7 @@ constexpr int bdf_waves_per_eu() {
   return NS <= 16 ? HPO_WAVES_PER_EU_SMALL : (LU32 ? HPO_WAVES_PER_EU_FP32 : HPO_WAVES_PER_EU_FP64);
 }
 
-template <int NS, bool LU32>
+template <int NS, bool LU32, bool PROF>
 __global__ __launch_bounds__(256) __attribute__((
     amdgpu_waves_per_eu(bdf_waves_per_eu<NS, LU32>(), bdf_waves_per_eu<NS, LU32>()))) void bdf_kernel(BdfArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -702,7 +702,7 @@ __global__ __launch_bounds__(256) __attribute__((
     c = __builtin_amdgcn_readfirstlane(c);
     if (static_cast<int64_t>(c) >= A.ncells) break;
     int64_t cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
-    integrate_cell<NS, LU32>(A, t, w, g, S, cell, lane);
+    integrate_cell<NS, LU32, PROF>(A, t, w, g, S, cell, lane);
   }
 }
 
@@ -756,17 +756,24 @@ __global__ __launch_bounds__(64) void lu_kernel(LuArgs A) {
 // ---------------------------------------------------------------------------
 template <int NS>
 static hipError_t launch_bdf_ns(const BdfArgs& A, int lu32, int grid, int block, size_t shmem, hipStream_t st) {
-  if (lu32)
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, true>), dim3(grid), dim3(block), shmem, st, A);
+  const bool prof = A.prof != nullptr;
+  if (lu32 && prof)
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, true>), dim3(grid), dim3(block), shmem, st, A);
+  else if (lu32)
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, false>), dim3(grid), dim3(block), shmem, st, A);
+  else if (prof)
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, false, true>), dim3(grid), dim3(block), shmem, st, A);
   else
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, false>), dim3(grid), dim3(block), shmem, st, A);
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, false, false>), dim3(grid), dim3(block), shmem, st, A);
   return hipGetLastError();
 }
 
 template <int NS>
-static const void* bdf_fn(int lu32) {
-  return lu32 ? reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true>)
-              : reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false>);
+static const void* bdf_fn(int lu32, int prof) {
+  if (lu32) return prof ? reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, true>)
+                        : reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, false>);
+  return prof ? reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false, true>)
+              : reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false, false>);
 }
 
 #define HPO_FOR_EACH_NS(X) X(9) X(10) X(53)
@@ -783,9 +790,9 @@ bool kernel_supports(int ns) {
   return false;
 }
 
-const void* bdf_kernel_fn(int ns, int lu32) {
+const void* bdf_kernel_fn(int ns, int lu32, int prof) {
 #define HPO_CASE(N) \
-  if (ns == N) return bdf_fn<N>(lu32);
+  if (ns == N) return bdf_fn<N>(lu32, prof);
   HPO_FOR_EACH_NS(HPO_CASE)
 #undef HPO_CASE
   return nullptr;

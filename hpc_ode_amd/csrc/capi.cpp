@@ -123,7 +123,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   const int block = 64 * wpb;
   const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) +
                      static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
-  const void* fn = hpo::bdf_kernel_fn(ns, o.lu_precision);
+  const void* fn = hpo::bdf_kernel_fn(ns, o.lu_precision, d_prof != nullptr);
   int occ = 0;
   HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
   if (occ < 1) throw std::invalid_argument("kernel does not fit on a CU with this configuration");

@@ -65,7 +65,7 @@ bool kernel_supports(int ns);
 int bdf_wave_lds_doubles(const DevMechLayout& L);
 int eval_wave_lds_doubles(const DevMechLayout& L);
 int64_t wave_scratch_doubles_host(const DevMechLayout& L);
-const void* bdf_kernel_fn(int ns, int lu32);
+const void* bdf_kernel_fn(int ns, int lu32, int prof = 0);
 hipError_t launch_bdf(int ns, const BdfArgs& A, int lu32, int grid, int block, size_t shmem, hipStream_t st);
 hipError_t launch_eval(int ns, const EvalArgs& A, int grid, int block, size_t shmem, hipStream_t st);
 hipError_t launch_lu(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st);
