@@ -664,20 +664,16 @@ HPO_DI float pivot_key(float v, int pos) {
 HPO_DI int key_pos(double k) { return 63 - static_cast<int>(__double_as_longlong(k) & 63); }
 HPO_DI int key_pos(float k) { return 63 - (__float_as_int(k) & 63); }
 
-// Row layout of the factors. FP32: after the elimination every row moves to the lane of its
-// elimination step (one lane permutation per register), so the triangular solves broadcast
-// from fixed lanes; this pays off because the FP32 path solves about twice per Newton
-// iteration (the solve and its refinement). FP64: rows stay in their lanes and the solves
-// look up the pivot lane of each step (the permutation of 106 registers costs more than it
-// saves at one solve per iteration).
-template <typename F>
-constexpr bool kLuPermuted = sizeof(F) == 4;
-
+// Row layout of the factors: after the elimination every row moves to the lane of its
+// elimination step (one lane permutation per register), so both triangular solves broadcast
+// from fixed lanes. Lane k then holds row k of the factors in pivot order: a[j < k] = L[k][j]
+// (unit lower triangle), a[j > k] = U[k][j] / U[k][k] (upper triangle scaled to a unit
+// diagonal), and rdiag = 1 / U[k][k]. The scaled upper triangle lets the back substitution
+// read each x_k with one broadcast and no multiply.
 template <int NS, typename F>
 struct LuReg {
   F a[NS];
-  F rdiag;    // 1/U_kk: permuted layout, lane k; in place, the lane that pivoted at step k
-  int step;   // in place: elimination step at which this lane was the pivot (64: never)
+  F rdiag;    // lane k: 1 / U[k][k]
   int perm;   // lane k: the lane (original row) that pivoted at step k
 };
 
@@ -685,15 +681,17 @@ HPO_DI float shfl(float v, int src) { return __shfl(v, src, 64); }
 
 // Branch-free over the elimination steps: a zero or NaN pivot column marks the matrix
 // singular and turns the remaining updates into no-ops instead of leaving the loop, so the
-// unrolled elimination has a single exit.
+// unrolled elimination has a single exit. The lane index is re-hidden from the optimiser on
+// entry so the per-step lane comparisons are formed here, not hoisted out of the caller's
+// loops into dozens of long-lived masks.
 template <int NS, typename F>
-HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
+HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane_in) {
+  const int lane = opaque(lane_in);
   int pos = lane;
   bool done = false;
   bool singular = false;
-  L.step = 64;
-  L.perm = 0;
-  L.rdiag = F(0);
+  F rd = F(0);
+  int perm = 0;
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     const bool cand = (!done) && (lane < NS);
@@ -711,11 +709,10 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
       if (lane == p) {
         pos = k;
         done = true;
-        L.step = k;
-        L.rdiag = mult;
+        rd = mult;
       }
     }
-    if (lane == k) L.perm = p;
+    if (lane == k) perm = p;
     F l = done ? F(0) : L.a[k] * mult;
     if (!done) L.a[k] = l;
 #pragma unroll
@@ -724,56 +721,43 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane) {
       L.a[j] = fma(-l, pj, L.a[j]);
     }
   }
-  if constexpr (kLuPermuted<F>) {
-    const bool live = lane < NS;
-    const int src = live ? L.perm : lane;
+  const bool live = lane < NS;
+  const int src = live ? perm : lane;
 #pragma unroll
-    for (int j = 0; j < NS; ++j) L.a[j] = live ? shfl(L.a[j], src) : F(0);
-    L.rdiag = live ? shfl(L.rdiag, src) : F(0);
+  for (int j = 0; j < NS; ++j) {
+    F v = live ? shfl(L.a[j], src) : F(0);
+    L.a[j] = v;
   }
+  rd = live ? shfl(rd, src) : F(0);
+#pragma unroll
+  for (int j = 1; j < NS; ++j) L.a[j] = (lane < j) ? L.a[j] * rd : L.a[j];
+  L.rdiag = rd;
+  L.perm = perm;
   return !singular;
 }
 
 // Solves (L U) x = P b for the factors of lu_factor; b and x are one entry per lane in the
 // original row / column order.
 template <int NS, typename F>
-HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane) {
-  if constexpr (kLuPermuted<F>) {
-    b = shfl(b, (lane < NS) ? L.perm : lane);
+HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane_in) {
+  const int lane = opaque(lane_in);
+  const bool live = lane < NS;
+  b = shfl(b, live ? L.perm : lane);
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      F yk = rl(b, k);
-      F c = (lane > k) ? L.a[k] : F(0);
-      b = fma(-c, yk, b);
-    }
-    F x = F(0);
-#pragma unroll
-    for (int k = NS - 1; k >= 0; --k) {
-      F xk = rl(b, k) * rl(L.rdiag, k);
-      x = (lane == k) ? xk : x;
-      F c = (lane < k) ? L.a[k] : F(0);
-      b = fma(-c, xk, b);
-    }
-    return x;
-  } else {
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      int p = rli(L.perm, k);
-      F yk = rl(b, p);
-      F c = (L.step > k) ? L.a[k] : F(0);
-      b = fma(-c, yk, b);
-    }
-    F x = F(0);
-#pragma unroll
-    for (int k = NS - 1; k >= 0; --k) {
-      int p = rli(L.perm, k);
-      F xk = rl(b, p) * rl(L.rdiag, p);
-      x = (lane == k) ? xk : x;
-      F c = (L.step < k) ? L.a[k] : F(0);
-      b = fma(-c, xk, b);
-    }
-    return x;
+  for (int k = 0; k < NS - 1; ++k) {
+    F yk = rl(b, k);
+    F c = (lane > k) ? L.a[k] : F(0);
+    b = fma(-c, yk, b);
   }
+  // U' x = D^-1 y with the unit-diagonal U' = D^-1 U
+  b *= L.rdiag;
+#pragma unroll
+  for (int k = NS - 1; k >= 1; --k) {
+    F xk = rl(b, k);
+    F c = (lane < k) ? L.a[k] : F(0);
+    b = fma(-c, xk, b);
+  }
+  return live ? b : F(0);
 }
 
 }  // namespace dev

@@ -22,7 +22,7 @@
 #define HPO_WAVES_PER_EU_FP64 1
 #endif
 #ifndef HPO_WAVES_PER_EU_FP32
-#define HPO_WAVES_PER_EU_FP32 1
+#define HPO_WAVES_PER_EU_FP32 2
 #endif
 
 namespace hpo {
@@ -504,10 +504,14 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           const double* Jrow = g.J + opaque(lane);
           for (int sw = 0; sw < A.refine_sweeps; ++sw) {
             // r = b - (I - gamma J) x in FP64, with the FP64 Jacobian row kept in this
-            // wave's scratch (cache-resident) rather than in registers
+            // wave's scratch (cache-resident) rather than in registers. x is broadcast through
+            // per-wave LDS (the enthalpy slots, which only the Jacobian uses, after writing them)
+            wave_sync();
+            w.hRT[lane] = x;
+            wave_sync();
             double Jx = 0.0;
 #pragma unroll 8
-            for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j * 64], rl(x, j), Jx);
+            for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j * 64], w.hRT[j], Jx);
             double r = b - x + gl * Jx;
             float d32 = lu_solve<NS>(lu.L, static_cast<float>(r), lane);
             x += static_cast<double>(d32);
