@@ -6,7 +6,10 @@
 The kernels are compiled with packed FP32 disabled (the packed-fp32-ops target
 feature off) and with machine-level loop-invariant code motion off: the latter
 otherwise hoists the FP64 polynomial constants of exp/log out of the persistent
-cell loop, where they stay live and push the BDF kernel into scratch spills.
+cell loop, where they stay live and push the BDF kernel into scratch spills. The
+pragma-unroll threshold is raised so the register-resident 53-step eliminations
+(2,809 row updates for the Gauss-Jordan inverse) unroll completely: a rolled loop
+would index the register row dynamically and move it to scratch memory.
 """
 from __future__ import annotations
 
@@ -29,7 +32,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 COMMON = ["-std=c++17", "-fPIC", "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-mllvm", "-disable-machine-licm", "-x", "hip"]
+             "-mllvm", "-disable-machine-licm", "-mllvm", "-pragma-unroll-threshold=1000000", "-x", "hip"]
 HOST_FLAGS = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-O2", "-Wall", "-Wno-unused-result"]
 
 SOURCES = [

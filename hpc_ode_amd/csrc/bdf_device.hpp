@@ -31,6 +31,11 @@ HPO_DI double rl(double v, int lane) {
 }
 HPO_DI float rl(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
 HPO_DI int rli(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+// v_writelane: lane `lane` of `old` takes the wave-uniform value v; other lanes keep theirs.
+HPO_DI int wli(int old, int v, int lane) {
+  asm("v_writelane_b32 %0, %1, %2" : "+v"(old) : "s"(v), "i"(lane));
+  return old;
+}
 
 template <int CTRL>
 HPO_DI double dpp(double v) {
@@ -85,6 +90,11 @@ HPO_DI int opaque(int v) {
   asm volatile("" : "+v"(v));
   return v;
 }
+
+// A condition every lane evaluates identically (on wave-uniform controller values held in
+// LDS), made visibly uniform: branches on it are scalar, and the counters and flags updated
+// under it stay in scalar registers instead of occupying a vector register each.
+HPO_DI bool ub(bool c) { return __builtin_amdgcn_readfirstlane(static_cast<int>(c)) != 0; }
 
 // Orders LDS and scratch accesses of the lanes of one wave (no workgroup barrier).
 HPO_DI void wave_sync() {
@@ -217,7 +227,8 @@ struct SpState {
   double Yk, invW, W, cpR, hRT, sR, dcpR;  // per lane (species k = lane)
 };
 
-HPO_DI void species_state(const Tabs& t, double y, double P, int lane, SpState& s) {
+HPO_DI void species_state(const Tabs& t, double y, double P, int lane_in, SpState& s) {
+  const int lane = opaque(lane_in);
   const int K = t.K;
   const double T = rl(y, 0);
   s.T = T;
@@ -266,7 +277,8 @@ HPO_DI double arrh(double lnA, double b, double th, bool constk, double lnT, dou
 // Each lane walks its share of the nonzeros, flushing partial sums per species run;
 // lane k then adds the partials of species k.
 template <int NSRC>
-HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part, int lane, double (&out)[NSRC]) {
+HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part, int lane_in, double (&out)[NSRC]) {
+  const int lane = opaque(lane_in);
   double acc[NSRC];
 #pragma unroll
   for (int s = 0; s < NSRC; ++s) acc[s] = 0.0;
@@ -372,7 +384,9 @@ HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, i
 // ---------------------------------------------------------------------------
 // RHS: constant-pressure dydt (src/Mechanism/CPU/dydt.c:7-133 in table form)
 // ---------------------------------------------------------------------------
-HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane, int n) {
+HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane_in, int n) {
+  // lane-derived addresses and masks are formed per call (see opaque)
+  const int lane = opaque(lane_in);
   const int K = t.K, R = t.R;
   SpState s;
   species_state(t, y, P, lane, s);
@@ -432,7 +446,8 @@ HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane,
 // ---------------------------------------------------------------------------
 // The row of lane l is written to g.J[j*64 + l], j = 0..NS-1.
 template <int NS>
-HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, double y, double P, int lane) {
+HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, double y, double P, int lane_in) {
+  const int lane = opaque(lane_in);
   const int K = t.K, R = t.R;
   SpState s;
   species_state(t, y, P, lane, s);
@@ -712,7 +727,7 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane_in) {
         rd = mult;
       }
     }
-    if (lane == k) perm = p;
+    perm = wli(perm, p, k);
     F l = done ? F(0) : L.a[k] * mult;
     if (!done) L.a[k] = l;
 #pragma unroll
@@ -737,7 +752,9 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane_in) {
 }
 
 // Solves (L U) x = P b for the factors of lu_factor; b and x are one entry per lane in the
-// original row / column order.
+// original row / column order. Each substitution step updates every lane without a lane mask:
+// the entry finished at step k is first copied into the result with v_writelane, so the
+// meaningless updates of already-finished lanes (and of the lane itself) are never read.
 template <int NS, typename F>
 HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane_in) {
   const int lane = opaque(lane_in);
@@ -758,6 +775,88 @@ HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane_in) {
     b = fma(-c, xk, b);
   }
   return live ? b : F(0);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 Newton matrix: Gauss-Jordan inverse with the same partial pivoting as lu_factor
+// (LAPACK's first-maximum choice among the rows not yet pivoted, so the pivot sequence is
+// the LU's), one row per lane. A solve is then a matrix-vector product whose operand is
+// broadcast through LDS: no serial substitution chain.
+// ---------------------------------------------------------------------------
+// Unscaled in-place elimination: at step k every other row (earlier pivot rows included)
+// subtracts l = a[k] / pivot times the pivot row, the pivot row keeps its values, and column
+// k takes the inverse's entries (-l; 1 in the pivot row). At the end each row is divided by
+// its own pivot and moved to the lane of its step, so lane k holds row k of (P M)^-1 and
+// M^-1 b = (P M)^-1 (P b).
+template <int NS>
+HPO_DI bool gj_invert(LuReg<NS, float>& L, int lane_in) {
+  const int lane = opaque(lane_in);
+  int pos = lane;
+  bool done = false;
+  bool singular = false;
+  float rd = 0.f;
+  int perm = 0;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const bool cand = (!done) && (lane < NS);
+    float key = cand ? pivot_key(L.a[k], pos) : 0.f;
+    float kmax = wmax(key);
+    int ppos = key_pos(kmax);
+    unsigned long long bal = __ballot(cand && pos == ppos);
+    int p = bal ? (__ffsll(bal) - 1) : 0;
+    float piv = rl(L.a[k], p);
+    bool bad = (bal == 0ull) || (piv == 0.f);
+    singular = singular || bad;
+    float mult = bad ? 0.f : 1.f / piv;
+    const bool isp = lane == p;
+    if (!bad) {
+      if (pos == k) pos = ppos;
+      if (isp) {
+        pos = k;
+        done = true;
+        rd = mult;
+      }
+    }
+    perm = wli(perm, p, k);
+    const float l = isp ? 0.f : L.a[k] * mult;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      if (j == k) continue;
+      float pj = rl(L.a[j], p);
+      L.a[j] = fma(-l, pj, L.a[j]);
+    }
+    L.a[k] = isp ? 1.f : -l;
+  }
+  const bool live = lane < NS;
+  const int src = live ? perm : lane;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    float v = L.a[j] * rd;
+    L.a[j] = live ? shfl(v, src) : 0.f;
+  }
+  L.rdiag = 0.f;
+  L.perm = perm;
+  return !singular;
+}
+
+// x = M^-1 b for the inverse of gj_invert; b and x are one entry per lane in the original
+// order. buf: 64 floats of this wave's LDS.
+template <int NS>
+HPO_DI float gj_apply(const LuReg<NS, float>& L, float b, int lane_in, float* buf) {
+  const int lane = opaque(lane_in);
+  const bool live = lane < NS;
+  const float bp = shfl(b, live ? L.perm : lane);
+  wave_sync();
+  buf[lane] = bp;
+  wave_sync();
+  float x0 = 0.f, x1 = 0.f;
+#pragma unroll
+  for (int j = 0; j + 1 < NS; j += 2) {
+    x0 = fma(L.a[j], buf[j], x0);
+    x1 = fma(L.a[j + 1], buf[j + 1], x1);
+  }
+  if (NS & 1) x0 = fma(L.a[NS - 1], buf[NS - 1], x0);
+  return live ? x0 + x1 : 0.f;
 }
 
 }  // namespace dev

@@ -21,6 +21,11 @@
 #ifndef HPO_WAVES_PER_EU_FP64
 #define HPO_WAVES_PER_EU_FP64 1
 #endif
+// Loads of the refinement's FP64 Jacobian row issued together (each group is one memory
+// round trip; more registers in flight per group)
+#ifndef HPO_JX_UNROLL
+#define HPO_JX_UNROLL 27
+#endif
 #ifndef HPO_WAVES_PER_EU_FP32
 #define HPO_WAVES_PER_EU_FP32 2
 #endif
@@ -480,7 +485,11 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           for (int j = 0; j < NS; ++j) lu.L.a[j] = -gamma * Jrow[j * 64] + ((j == ol) ? 1.0 : 0.0);
         }
         jcur = jnew ? 1 : 0;
-        bool ok = lu_factor<NS>(lu.L, lane);
+        bool ok;
+        if constexpr (LU32)
+          ok = gj_invert<NS>(lu.L, lane);
+        else
+          ok = lu_factor<NS>(lu.L, lane);
         if (prof) c_lu += clock64() - c_a;
         S.gamma_lu = gamma;
         nsetups++;
@@ -498,10 +507,11 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         double x;
         if (prof) c_a = clock64();
         if constexpr (LU32) {
-          float x32 = lu_solve<NS>(lu.L, static_cast<float>(b), lane);
+          const double* Jrow = g.J + opaque(lane);
+          float* vbuf = reinterpret_cast<float*>(w.cp);
+          float x32 = gj_apply<NS>(lu.L, static_cast<float>(b), lane, vbuf);
           x = static_cast<double>(x32);
           const double gl = S.gamma_lu;
-          const double* Jrow = g.J + opaque(lane);
           for (int sw = 0; sw < A.refine_sweeps; ++sw) {
             // r = b - (I - gamma J) x in FP64, with the FP64 Jacobian row kept in this
             // wave's scratch (cache-resident) rather than in registers. x is broadcast through
@@ -510,10 +520,10 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
             w.hRT[lane] = x;
             wave_sync();
             double Jx = 0.0;
-#pragma unroll 8
+#pragma unroll HPO_JX_UNROLL
             for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j * 64], w.hRT[j], Jx);
             double r = b - x + gl * Jx;
-            float d32 = lu_solve<NS>(lu.L, static_cast<float>(r), lane);
+            float d32 = gj_apply<NS>(lu.L, static_cast<float>(r), lane, vbuf);
             x += static_cast<double>(d32);
           }
         } else {
@@ -746,9 +756,17 @@ __global__ __launch_bounds__(64) void lu_kernel(LuArgs A) {
   LuReg<NS, F> L;
 #pragma unroll
   for (int j = 0; j < NS; ++j) L.a[j] = lane < NS ? static_cast<F>(A.M[(c * NS + lane) * NS + j]) : F(0);
-  bool ok = lu_factor<NS>(L, lane);
+  bool ok;
+  F x;
   F b = lane < NS ? static_cast<F>(A.b[c * NS + lane]) : F(0);
-  F x = lu_solve<NS>(L, b, lane);
+  if constexpr (sizeof(F) == 4) {
+    __shared__ float buf[64];
+    ok = gj_invert<NS>(L, lane);
+    x = gj_apply<NS>(L, b, lane, buf);
+  } else {
+    ok = lu_factor<NS>(L, lane);
+    x = lu_solve<NS>(L, b, lane);
+  }
   if (lane < NS) A.x[c * NS + lane] = static_cast<double>(x);
   if (lane == 0) A.ok[c] = ok ? 1 : 0;
 }
