@@ -788,6 +788,28 @@ HPO_DI F lu_solve(const LuReg<NS, F>& L, F b, int lane_in) {
 // k takes the inverse's entries (-l; 1 in the pivot row). At the end each row is divided by
 // its own pivot and moved to the lane of its step, so lane k holds row k of (P M)^-1 and
 // M^-1 b = (P M)^-1 (P b).
+//
+// The pivot search is an unsigned maximum over 32-bit keys (|a| with the row position in the
+// low six bits): integer maxima need no NaN canonicalisation and fuse with their DPP moves.
+HPO_DI unsigned pivot_key_u(float v, int pos) {
+  return (static_cast<unsigned>(__float_as_int(fabsf(v))) & ~63u) | static_cast<unsigned>(63 - pos);
+}
+template <int CTRL>
+HPO_DI unsigned dpp_u(unsigned v) {
+  return static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+}
+HPO_DI unsigned wmax_u(unsigned v) {
+  v = max(v, dpp_u<0xB1>(v));
+  v = max(v, dpp_u<0x4E>(v));
+  v = max(v, dpp_u<0x141>(v));
+  v = max(v, dpp_u<0x140>(v));
+  const unsigned a = static_cast<unsigned>(rli(static_cast<int>(v), 0));
+  const unsigned b = static_cast<unsigned>(rli(static_cast<int>(v), 16));
+  const unsigned c = static_cast<unsigned>(rli(static_cast<int>(v), 32));
+  const unsigned d = static_cast<unsigned>(rli(static_cast<int>(v), 48));
+  return max(max(a, b), max(c, d));
+}
+
 template <int NS>
 HPO_DI bool gj_invert(LuReg<NS, float>& L, int lane_in) {
   const int lane = opaque(lane_in);
@@ -799,13 +821,13 @@ HPO_DI bool gj_invert(LuReg<NS, float>& L, int lane_in) {
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     const bool cand = (!done) && (lane < NS);
-    float key = cand ? pivot_key(L.a[k], pos) : 0.f;
-    float kmax = wmax(key);
-    int ppos = key_pos(kmax);
+    const unsigned key = cand ? pivot_key_u(L.a[k], pos) : 0u;
+    const unsigned kmax = wmax_u(key);
+    const int ppos = 63 - static_cast<int>(kmax & 63u);
     unsigned long long bal = __ballot(cand && pos == ppos);
     int p = bal ? (__ffsll(bal) - 1) : 0;
     float piv = rl(L.a[k], p);
-    bool bad = (bal == 0ull) || (piv == 0.f);
+    bool bad = (bal == 0ull) || !(fabsf(piv) > 0.f);
     singular = singular || bad;
     float mult = bad ? 0.f : 1.f / piv;
     const bool isp = lane == p;
