@@ -23,7 +23,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.environ.get("HPO_CSRC") or os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
-BUILD = os.path.join(HERE, "_build")
+BUILD = os.environ.get("HPO_BUILD_DIR") or os.path.join(HERE, "_build")
 LIB = os.environ.get("HPO_LIB") or os.path.join(HERE, "libhpc_ode.so")
 EXTRA = os.environ.get("HPO_EXTRA_FLAGS", "").split()
 ARCH = os.environ.get("HPO_OFFLOAD_ARCH", "gfx950")
