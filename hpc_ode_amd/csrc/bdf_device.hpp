@@ -282,19 +282,36 @@ HPO_DI void gather(const Tabs& t, const double* const (&src)[NSRC], double* part
   double acc[NSRC];
 #pragma unroll
   for (int s = 0; s < NSRC; ++s) acc[s] = 0.0;
-#pragma unroll 4
-  for (int i = 0; i < t.ngl; ++i) {
-    uint32_t e = t.gather[i * 64 + lane];
-    double c = static_cast<double>(static_cast<int32_t>(e << 16) >> 28);
-    int r = static_cast<int>(e & 4095u);
-    // flush the partial sum at the end of a species run; other entries write a scratch slot
-    const bool flush = (e >> 16) & 1u;
-    const int p = flush ? static_cast<int>(e >> 17) : t.npart;
+  // Entries are read in chunks: the table words and source values of a chunk are all loaded
+  // before the chunk's partial sums are stored, so the loads are not serialised behind stores
+  // the compiler cannot prove independent. Entries past the table end read as padding
+  // (reaction 0, coefficient 0, no flush).
+  constexpr int CH = 4;
+#pragma unroll 1
+  for (int i0 = 0; i0 < t.ngl; i0 += CH) {
+    uint32_t ev[CH];
+    double v[CH][NSRC];
 #pragma unroll
-    for (int s = 0; s < NSRC; ++s) {
-      acc[s] += c * src[s][r];
-      part[s * (t.npart + 1) + p] = acc[s];
-      acc[s] = flush ? 0.0 : acc[s];
+    for (int u = 0; u < CH; ++u) ev[u] = (i0 + u < t.ngl) ? t.gather[(i0 + u) * 64 + lane] : 0u;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int r = static_cast<int>(ev[u] & 4095u);
+#pragma unroll
+      for (int s = 0; s < NSRC; ++s) v[u][s] = src[s][r];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const uint32_t e = ev[u];
+      const double c = static_cast<double>(static_cast<int32_t>(e << 16) >> 28);
+      // flush the partial sum at the end of a species run; other entries write a scratch slot
+      const bool flush = (e >> 16) & 1u;
+      const int p = flush ? static_cast<int>(e >> 17) : t.npart;
+#pragma unroll
+      for (int s = 0; s < NSRC; ++s) {
+        acc[s] += c * v[u][s];
+        part[s * (t.npart + 1) + p] = acc[s];
+        acc[s] = flush ? 0.0 : acc[s];
+      }
     }
   }
   wave_sync();
@@ -592,17 +609,29 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
   wave_sync();
   // owner-computes assembly of d wdot / dC (rows 0..K-2) and sum_k H_k d wdot_k / dC (row K)
   {
+    // chunks of terms: all loads of a chunk (term words, then coefficient and slot values)
+    // are issued before its stores. Terms past the table end read as coefficient 0, slot 0,
+    // no flush; every lane's list ends with a flushed term, so they are never stored.
+    constexpr int CH = 8;
     double acc = 0.0;
 #pragma unroll 1
-    for (int i = 0; i < t.njl; ++i) {
-      uint64_t e = t.jterm[i * 64 + lane];
-      int cell = static_cast<int>(e & 0xFFFFu);
-      int src = static_cast<int>((e >> 16) & 0xFFFFu);
-      int ci = static_cast<int>((e >> 32) & 0xFFFFu);
-      acc += g.coefbuf[ci] * g.dphi[src];
-      if ((e >> 48) & 1u) {
-        g.JC[cell] = acc;
-        acc = 0.0;
+    for (int i0 = 0; i0 < t.njl; i0 += CH) {
+      uint64_t ev[CH];
+      double cv[CH], dv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) ev[u] = (i0 + u < t.njl) ? t.jterm[(i0 + u) * 64 + lane] : 0ull;
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        cv[u] = g.coefbuf[static_cast<int>((ev[u] >> 32) & 0xFFFFu)];
+        dv[u] = g.dphi[static_cast<int>((ev[u] >> 16) & 0xFFFFu)];
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        acc += cv[u] * dv[u];
+        if ((ev[u] >> 48) & 1u) {
+          g.JC[static_cast<int>(ev[u] & 0xFFFFu)] = acc;
+          acc = 0.0;
+        }
       }
     }
   }
