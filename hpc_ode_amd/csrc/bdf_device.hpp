@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fastmath.hpp"
 #include "mechanism.hpp"
 
 namespace hpo {
@@ -234,7 +235,7 @@ HPO_DI void species_state(const Tabs& t, double y, double P, int lane_in, SpStat
   s.T = T;
   s.P = P;
   s.invT = 1.0 / T;
-  s.lnT = log(T);
+  s.lnT = fast_log(T);
   double Yk = shfl(y, (lane + 1) & 63);
   const double* rec = t.sp + (lane < K ? lane : K - 1) * kSpStride;
   s.invW = rec[0];
@@ -385,8 +386,8 @@ HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, i
       if (h.kind == kTroe) {
         double Fc = (1.0 - pr.troe_a) * exp(-s.T * pr.troe_iT3) + pr.troe_a * exp(-s.T * pr.troe_iT1) +
                     ((h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0);
-        double logFc = log10(fmax(Fc, 1.0e-300));
-        double x = log10(fmax(Pr, 1.0e-300));
+        double logFc = fast_log10(fmax(Fc, 1.0e-300));
+        double x = fast_log10(fmax(Pr, 1.0e-300));
         double A = x - 0.67 * logFc - 0.4;
         double B = 0.806 - 1.1762 * logFc - 0.14 * x;
         const double B2 = B * B;
@@ -554,8 +555,8 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
           double dFc = -(1.0 - pr.troe_a) * pr.troe_iT3 * e3 - pr.troe_a * pr.troe_iT1 * e1 +
                        ((h.flags & 4) ? pr.troe_T2 * s.invT * s.invT * e2 : 0.0);
           bool fcl = !(Fc > 1.0e-300), prl = !(Pr > 1.0e-300);
-          double logFc = log10(fmax(Fc, 1.0e-300));
-          double x = log10(fmax(Pr, 1.0e-300));
+          double logFc = fast_log10(fmax(Fc, 1.0e-300));
+          double x = fast_log10(fmax(Pr, 1.0e-300));
           double A = x - 0.67 * logFc - 0.4;
           double B = 0.806 - 1.1762 * logFc - 0.14 * x;
           double D = 1.0 + A * A / (B * B);

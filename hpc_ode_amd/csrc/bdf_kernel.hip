@@ -322,7 +322,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
       }
       S.etamax = 1.0;
       if (nef <= MXNEF1) {
-        double eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+        double eta = 1.0 / (fast_pow(BIAS2 * dsm, 1.0 / L) + ADDON);
         eta = fmax(ETAMIN, eta);
         if (nef >= SMALL_NEF) eta = fmin(eta, ETAMXF);
         S.eta = eta;
@@ -367,7 +367,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
       S.hprime = h;
       S.eta = 1.0;
     } else {
-      double etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+      double etaq = 1.0 / (fast_pow(BIAS2 * dsm, 1.0 / L) + ADDON);
       double eta;
       if (qwait != 0) {
         eta = etaq;
@@ -377,7 +377,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         double etaqm1 = 0.0, etaqp1 = 0.0;
         if (q > 1) {
           double ddn = wrms(pick(zn, q), ewt, n) * S.tq[1];
-          etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / q) + ADDON);
+          etaqm1 = 1.0 / (fast_pow(BIAS1 * ddn, 1.0 / q) + ADDON);
         }
         if (q != QMAX && S.saved_tq5 != 0.0) {
           double rr = h / S.tau[2], pw = 1.0;
@@ -385,7 +385,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           double cquot = (S.tq[5] / S.saved_tq5) * pw;
           double tempv = -cquot * zn[QMAX] + acor;
           double dup = wrms(tempv, ewt, n) * S.tq[3];
-          etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+          etaqp1 = 1.0 / (fast_pow(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
         }
         double etam = fmax(etaqm1, fmax(etaq, etaqp1));
         if (etam < THRESH) {
