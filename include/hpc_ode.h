@@ -49,7 +49,8 @@ typedef struct {
   double rtol;             /* scalar relative tolerance (CVodeSStolerances) */
   double atol;             /* scalar absolute tolerance */
   int32_t maxsteps;        /* CVodeSetMaxNumSteps; reference uses 10000 */
-  int32_t lu_precision;    /* 0: FP64 Newton-matrix LU; 1: FP32 LU + FP64 residual refinement */
+  int32_t lu_precision;    /* 0: FP64 Newton-matrix LU; 1: FP32 Newton matrix (Gauss-Jordan inverse
+                              with the LU's partial pivoting) + FP64 residual refinement */
   int32_t refine_sweeps;   /* refinement sweeps per linear solve when lu_precision == 1 */
   int32_t waves_per_block; /* cells resident per workgroup (1..4); 0 = default */
   int32_t blocks_per_cu;   /* persistent workgroups per CU; 0 = as many as fit */
@@ -192,7 +193,9 @@ int hpo_server_hbm_used(const hpo_server *s, int64_t *used_bytes, int64_t *budge
 int hpo_eval_rhs(const hpo_mech *m, int64_t n, const double *y, const double *P, double *f);
 /* J = d f / d y, [n][n_state][n_state] row-major; what eval_jacob (jacob.c:3-371) computes. */
 int hpo_eval_jac(const hpo_mech *m, int64_t n, const double *y, const double *P, double *J);
-/* Batched dense LU + solve with the kernel's register LU: M [n][ns][ns] row-major, b, x [n][ns]. */
+/* Batched dense solve with the kernel's register-resident Newton-matrix code: M [n][ns][ns]
+ * row-major, b, x [n][ns]. fp32 = 0: FP64 partial-pivot LU and triangular solves; fp32 = 1: the
+ * FP32 Gauss-Jordan inverse (same pivots) applied as a matrix-vector product, no refinement. */
 int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double *M, const double *b, double *x,
                  int32_t *ok);
 
