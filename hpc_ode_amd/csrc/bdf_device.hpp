@@ -462,7 +462,8 @@ HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane_
 // ---------------------------------------------------------------------------
 // Analytic Jacobian (what src/Mechanism/CPU/jacob.c:3-371 computes), one row per lane.
 // ---------------------------------------------------------------------------
-// The row of lane l is written to g.J[j*64 + l], j = 0..NS-1.
+// The row of lane l < NS is written to g.J[j*NS + l], j = 0..NS-1 (columns packed at the
+// state length, so a column read moves NS doubles, not 64).
 template <int NS>
 HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, double y, double P, int lane_in) {
   const int lane = opaque(lane_in);
@@ -676,7 +677,7 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
     double dQdT = SCpw + SHJT - sigma * s.invT - (s.m * s.invT) * SHJm;  // lane 0: sigma = sum_k H_k sigma_k
     double dDdT = -rho * cpmix * s.invT + rho * dcpmix;
     double tv = (-dQdT - fT * dDdT) * invD;
-    g.J[lane] = live ? (isT ? tv : spv) : 0.0;
+    if (live) g.J[lane] = isT ? tv : spv;
   }
   const double c1 = jcN * rho * invWN;
 #pragma unroll 2
@@ -687,7 +688,7 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
     double spv = fac * (base + wdot_k * Wbar * dj);
     double dDdY = -rho * Wbar * dj * cpmix + rho * (w.cp[j] - cpN);
     double tv = (-base - fT * dDdY) * invD;
-    g.J[(j + 1) * 64 + lane] = live ? (isT ? tv : spv) : 0.0;
+    if (live) g.J[(j + 1) * NS + lane] = isT ? tv : spv;
   }
   wave_sync();
 }

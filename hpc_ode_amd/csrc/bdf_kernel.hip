@@ -470,7 +470,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           if (prof) c_a = clock64();
 #pragma unroll
           for (int j = 0; j < NS; ++j) {
-            double Jv = Jrow[j * 64];
+            double Jv = Jrow[j * NS];
             lu.L.a[j] = static_cast<float>(-gamma * Jv + ((j == ol) ? 1.0 : 0.0));
           }
         } else {
@@ -483,7 +483,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           }
           if (prof) c_a = clock64();
 #pragma unroll
-          for (int j = 0; j < NS; ++j) lu.L.a[j] = -gamma * Jrow[j * 64] + ((j == ol) ? 1.0 : 0.0);
+          for (int j = 0; j < NS; ++j) lu.L.a[j] = -gamma * Jrow[j * NS] + ((j == ol) ? 1.0 : 0.0);
         }
         jcur = jnew ? 1 : 0;
         bool ok;
@@ -522,7 +522,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
             wave_sync();
             double Jx = 0.0;
 #pragma unroll HPO_JX_UNROLL
-            for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j * 64], w.hRT[j], Jx);
+            for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j * NS], w.hRT[j], Jx);
             double r = b - x + gl * Jx;
             float d32 = gj_apply<NS>(lu.L, static_cast<float>(r), lane, vbuf);
             x += static_cast<double>(d32);
@@ -745,7 +745,7 @@ __global__ __launch_bounds__(256) void eval_kernel(EvalArgs A) {
     } else {
       jacobian<NS>(t, w, g, y, P, lane);
       if (lane < NS)
-        for (int j = 0; j < NS; ++j) A.out[(c * NS + lane) * NS + j] = g.J[j * 64 + lane];
+        for (int j = 0; j < NS; ++j) A.out[(c * NS + lane) * NS + j] = g.J[j * NS + lane];
     }
   }
 }
