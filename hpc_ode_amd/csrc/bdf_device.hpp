@@ -92,11 +92,6 @@ HPO_DI int opaque(int v) {
   return v;
 }
 
-// A condition every lane evaluates identically (on wave-uniform controller values held in
-// LDS), made visibly uniform: branches on it are scalar, and the counters and flags updated
-// under it stay in scalar registers instead of occupying a vector register each.
-HPO_DI bool ub(bool c) { return __builtin_amdgcn_readfirstlane(static_cast<int>(c)) != 0; }
-
 // Orders LDS and scratch accesses of the lanes of one wave (no workgroup barrier).
 HPO_DI void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");

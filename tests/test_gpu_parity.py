@@ -97,21 +97,28 @@ def test_lu_newton_matrices(gri, fp32):
     assert np.all(rel < 50 * cond * eps), (rel, cond)
 
 
-def test_lu_random_dense_pivoting():
-    """Random dense matrices need row exchanges; singular input is reported, not solved."""
+@pytest.mark.parametrize("fp32", [False, True])
+def test_lu_random_dense_pivoting(fp32):
+    """Random dense matrices need row exchanges; singular input is reported, not solved.
+    fp32: the FP32 path's Gauss-Jordan inverse with the same pivoting (error bound scaled to
+    single precision: 50 cond(M) eps32, as for the Newton matrices)."""
     rng = np.random.default_rng(7)
     n = 53
     M = rng.standard_normal((8, n, n))
     M[3] = np.triu(M[3])[::-1]           # forces a pivot in every column
     M[5, :, 7] = 0.0                     # singular
     b = rng.standard_normal((8, n))
-    x, ok = H.lu_solve(M, b)
+    x, ok = H.lu_solve(M, b, fp32=fp32)
     assert ok.tolist() == [1, 1, 1, 1, 1, 0, 1, 1]
     for i in range(8):
         if i == 5:
             continue
         ref = np.linalg.solve(M[i], b[i])
-        assert np.abs(x[i] - ref).max() <= 1e-9 * np.abs(ref).max() * np.linalg.cond(M[i]) ** 0.5
+        err = np.abs(x[i] - ref).max() / np.abs(ref).max()
+        if fp32:
+            assert err <= 50 * np.linalg.cond(M[i]) * np.finfo(np.float32).eps, (i, err)
+        else:
+            assert err <= 1e-9 * np.linalg.cond(M[i]) ** 0.5, (i, err)
 
 
 def _compare_states(yg, yo, T_rtol, w_max, rtol=1e-6, atol=1e-10):
