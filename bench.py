@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--sort", type=int, default=1)
+    ap.add_argument("--lu-layout", type=int, default=0, choices=(0, 1),
+                    help="FP32 Newton matrix: 0 explicit inverse (Gauss-Jordan), 1 in-place LU")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -220,7 +222,7 @@ def main():
         return wall / K * 1e3, float(np.mean(ker)), st, stats, t
 
     o32 = H.opts(lu_precision=1, refine_sweeps=a.refine_sweeps, waves_per_block=a.waves_per_block,
-                 blocks_per_cu=a.blocks_per_cu)
+                 blocks_per_cu=a.blocks_per_cu, lu_layout=a.lu_layout)
     tuned = None
     if tune:
         d_order = torch.empty(n, dtype=torch.int32, device=dev)
@@ -300,6 +302,7 @@ def main():
                    "autotune": tuned,
                    "cells": n, "mechanism": "gri30.yaml (53 species, 325 reactions)",
                    "rtol": 1e-6, "atol": 1e-10, "refine_sweeps": a.refine_sweeps,
+                   "lu_layout": H.LU_LAYOUTS[o32.lu_layout],
                    "grid": t32.grid, "block": t32.block, "lds_bytes": t32.lds_bytes},
         "failed_cells": int((st32 != 0).sum()),
         "fp64_plain": fp64,

@@ -35,7 +35,7 @@ class Opts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("maxsteps", C.c_int32),
                 ("lu_precision", C.c_int32), ("refine_sweeps", C.c_int32),
                 ("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("sort_cells", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("sort_cells", C.c_int32), ("lu_layout", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Timing(C.Structure):
@@ -46,17 +46,20 @@ class Timing(C.Structure):
 
 TUNE_MAX = 16
 ORDER_MODES = {0: "cell index", 1: "T0 descending", 2: "predicted cost descending"}
+LU_LAYOUTS = {0: "inverse", 1: "lu"}
 
 
 class TuneReport(C.Structure):
     _fields_ = [("n_candidates", C.c_int32), ("best", C.c_int32),
                 ("waves_per_block", C.c_int32 * TUNE_MAX), ("blocks_per_cu", C.c_int32 * TUNE_MAX),
-                ("order_mode", C.c_int32 * TUNE_MAX), ("sample_ms", C.c_double * TUNE_MAX),
+                ("order_mode", C.c_int32 * TUNE_MAX), ("lu_layout", C.c_int32 * TUNE_MAX),
+                ("sample_ms", C.c_double * TUNE_MAX),
                 ("sample_cells", C.c_int64), ("tune_ms", C.c_double)]
 
     def candidates(self):
         return [dict(waves_per_block=self.waves_per_block[i], blocks_per_cu=self.blocks_per_cu[i],
-                     order=ORDER_MODES[self.order_mode[i]], sample_ms=self.sample_ms[i])
+                     order=ORDER_MODES[self.order_mode[i]], lu_layout=LU_LAYOUTS[self.lu_layout[i]],
+                     sample_ms=self.sample_ms[i])
                 for i in range(self.n_candidates)]
 
 
@@ -137,12 +140,13 @@ def _p(a, t=_dp):
 
 
 def opts(rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0, refine_sweeps=1,
-         waves_per_block=0, blocks_per_cu=0, sort_cells=0) -> Opts:
+         waves_per_block=0, blocks_per_cu=0, sort_cells=0, lu_layout=0) -> Opts:
     o = Opts()
     lib().hpo_opts_default(C.byref(o))
     o.rtol, o.atol, o.maxsteps = rtol, atol, maxsteps
     o.lu_precision, o.refine_sweeps = lu_precision, refine_sweeps
     o.waves_per_block, o.blocks_per_cu, o.sort_cells = waves_per_block, blocks_per_cu, sort_cells
+    o.lu_layout = lu_layout
     return o
 
 
@@ -206,7 +210,7 @@ class Mechanism:
     # --- integration ------------------------------------------------------
     def integrate(self, y, P, dt, rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0,
                   refine_sweeps=1, waves_per_block=0, blocks_per_cu=0, sort_cells=0,
-                  return_timing=False):
+                  return_timing=False, lu_layout=0):
         """Integrates every row of y ([ncells, n_state]) from t=0 to dt on the GPU.
         Returns (y_out, status, stats[, timing])."""
         y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
@@ -214,7 +218,7 @@ class Mechanism:
         b = Batch(self, n)
         b.put(y, P, dt)
         t = b.integrate(n, opts(rtol, atol, maxsteps, lu_precision, refine_sweeps, waves_per_block,
-                                blocks_per_cu, sort_cells))
+                                blocks_per_cu, sort_cells, lu_layout))
         yo, st, stats = b.get(n)
         return (yo, st, stats, t) if return_timing else (yo, st, stats)
 
@@ -249,8 +253,8 @@ class Batch:
         return t
 
     def autotune(self, count, o: Opts, sample=32768, stream=None) -> TuneReport:
-        """Tunes launch shape and queue order for cells [0, count); o comes back tuned
-        (sort_cells = 2 selects the tuned order in integrate)."""
+        """Tunes launch shape, queue order and (FP32 path) Newton-matrix layout for cells
+        [0, count); o comes back tuned (sort_cells = 2 selects the tuned order in integrate)."""
         rep = TuneReport()
         _check(lib().hpo_batch_autotune(self._h, count, sample, C.byref(o), C.byref(rep), stream))
         return rep

@@ -887,6 +887,11 @@ HPO_DI bool gj_invert(LuReg<NS, float>& L, int lane_in) {
   return !singular;
 }
 
+// FP32 solve of the BDF kernel's Newton system in either layout: the explicit inverse of
+// gj_invert (buf: 64 floats of this wave's LDS) or the in-place factors of lu_factor.
+template <int NS, bool INV>
+HPO_DI float solve32(const LuReg<NS, float>& L, float b, int lane, float* buf);
+
 // x = M^-1 b for the inverse of gj_invert; b and x are one entry per lane in the original
 // order. buf: 64 floats of this wave's LDS.
 template <int NS>
@@ -905,6 +910,14 @@ HPO_DI float gj_apply(const LuReg<NS, float>& L, float b, int lane_in, float* bu
   }
   if (NS & 1) x0 = fma(L.a[NS - 1], buf[NS - 1], x0);
   return live ? x0 + x1 : 0.f;
+}
+
+template <int NS, bool INV>
+HPO_DI float solve32(const LuReg<NS, float>& L, float b, int lane, float* buf) {
+  if constexpr (INV)
+    return gj_apply<NS>(L, b, lane, buf);
+  else
+    return lu_solve<NS>(L, b, lane);
 }
 
 }  // namespace dev

@@ -78,7 +78,7 @@ struct Scal {
 static_assert(sizeof(Scal) % 8 == 0, "Scal layout");
 constexpr int kScalDoubles = sizeof(Scal) / 8;
 
-template <int NS, bool LU32, bool PROF>
+template <int NS, bool LU32, bool PROF, bool INV>
 __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w, const WaveScratch& g, Scal& S,
                                int64_t cell, int lane) {
   const int n = NS;
@@ -487,7 +487,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         }
         jcur = jnew ? 1 : 0;
         bool ok;
-        if constexpr (LU32)
+        if constexpr (INV)
           ok = gj_invert<NS>(lu.L, lane);
         else
           ok = lu_factor<NS>(lu.L, lane);
@@ -510,7 +510,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         if constexpr (LU32) {
           const double* Jrow = g.J + opaque(lane);
           float* vbuf = reinterpret_cast<float*>(w.cp);
-          float x32 = gj_apply<NS>(lu.L, static_cast<float>(b), lane, vbuf);
+          float x32 = solve32<NS, INV>(lu.L, static_cast<float>(b), lane, vbuf);
           x = static_cast<double>(x32);
           const double gl = S.gamma_lu;
           for (int sw = 0; sw < A.refine_sweeps; ++sw) {
@@ -524,7 +524,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
 #pragma unroll HPO_JX_UNROLL
             for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j * NS], w.hRT[j], Jx);
             double r = b - x + gl * Jx;
-            float d32 = gj_apply<NS>(lu.L, static_cast<float>(r), lane, vbuf);
+            float d32 = solve32<NS, INV>(lu.L, static_cast<float>(r), lane, vbuf);
             x += static_cast<double>(d32);
           }
         } else {
@@ -692,7 +692,7 @@ constexpr int bdf_waves_per_eu() {
   return NS <= 16 ? HPO_WAVES_PER_EU_SMALL : (LU32 ? HPO_WAVES_PER_EU_FP32 : HPO_WAVES_PER_EU_FP64);
 }
 
-template <int NS, bool LU32, bool PROF>
+template <int NS, bool LU32, bool PROF, bool INV>
 __global__ __launch_bounds__(256) __attribute__((
     amdgpu_waves_per_eu(bdf_waves_per_eu<NS, LU32>(), bdf_waves_per_eu<NS, LU32>()))) void bdf_kernel(BdfArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -717,7 +717,7 @@ __global__ __launch_bounds__(256) __attribute__((
     c = __builtin_amdgcn_readfirstlane(c);
     if (static_cast<int64_t>(c) >= A.ncells) break;
     int64_t cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
-    integrate_cell<NS, LU32, PROF>(A, t, w, g, S, cell, lane);
+    integrate_cell<NS, LU32, PROF, INV>(A, t, w, g, S, cell, lane);
   }
 }
 
@@ -777,26 +777,36 @@ __global__ __launch_bounds__(64) void lu_kernel(LuArgs A) {
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 // ---------------------------------------------------------------------------
-template <int NS>
-static hipError_t launch_bdf_ns(const BdfArgs& A, int lu32, int grid, int block, size_t shmem, hipStream_t st) {
-  const bool prof = A.prof != nullptr;
-  if (lu32 && prof)
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, true>), dim3(grid), dim3(block), shmem, st, A);
-  else if (lu32)
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, false>), dim3(grid), dim3(block), shmem, st, A);
-  else if (prof)
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, false, true>), dim3(grid), dim3(block), shmem, st, A);
-  else
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, false, false>), dim3(grid), dim3(block), shmem, st, A);
-  return hipGetLastError();
+// Kernel variants: 0 FP64 LU; 1 FP32 explicit inverse (Gauss-Jordan); 2 FP32 in-place LU.
+template <int NS, bool PROF>
+static const void* bdf_fn_p(int variant) {
+  if (variant == 1) return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, PROF, true>);
+  if (variant == 2) return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, PROF, false>);
+  return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false, PROF, false>);
 }
 
 template <int NS>
-static const void* bdf_fn(int lu32, int prof) {
-  if (lu32) return prof ? reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, true>)
-                        : reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, false>);
-  return prof ? reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false, true>)
-              : reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false, false>);
+static const void* bdf_fn(int variant, int prof) {
+  return prof ? bdf_fn_p<NS, true>(variant) : bdf_fn_p<NS, false>(variant);
+}
+
+template <int NS, bool PROF>
+static void launch_bdf_p(const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st) {
+  if (variant == 1)
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, PROF, true>), dim3(grid), dim3(block), shmem, st, A);
+  else if (variant == 2)
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, PROF, false>), dim3(grid), dim3(block), shmem, st, A);
+  else
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, false, PROF, false>), dim3(grid), dim3(block), shmem, st, A);
+}
+
+template <int NS>
+static hipError_t launch_bdf_ns(const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st) {
+  if (A.prof != nullptr)
+    launch_bdf_p<NS, true>(A, variant, grid, block, shmem, st);
+  else
+    launch_bdf_p<NS, false>(A, variant, grid, block, shmem, st);
+  return hipGetLastError();
 }
 
 #define HPO_FOR_EACH_NS(X) X(9) X(10) X(53)
@@ -813,17 +823,17 @@ bool kernel_supports(int ns) {
   return false;
 }
 
-const void* bdf_kernel_fn(int ns, int lu32, int prof) {
+const void* bdf_kernel_fn(int ns, int variant, int prof) {
 #define HPO_CASE(N) \
-  if (ns == N) return bdf_fn<N>(lu32, prof);
+  if (ns == N) return bdf_fn<N>(variant, prof);
   HPO_FOR_EACH_NS(HPO_CASE)
 #undef HPO_CASE
   return nullptr;
 }
 
-hipError_t launch_bdf(int ns, const BdfArgs& A, int lu32, int grid, int block, size_t shmem, hipStream_t st) {
+hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st) {
 #define HPO_CASE(N) \
-  if (ns == N) return launch_bdf_ns<N>(A, lu32, grid, block, shmem, st);
+  if (ns == N) return launch_bdf_ns<N>(A, variant, grid, block, shmem, st);
   HPO_FOR_EACH_NS(HPO_CASE)
 #undef HPO_CASE
   return hipErrorInvalidValue;

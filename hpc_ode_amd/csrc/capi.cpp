@@ -105,6 +105,7 @@ void check_opts(const hpo_opts& o) {
   if (o.refine_sweeps < 0 || o.refine_sweeps > 8) throw std::invalid_argument("refine_sweeps must be in 0..8");
   if (o.waves_per_block < 0 || o.waves_per_block > 4) throw std::invalid_argument("waves_per_block must be in 0..4");
   if (o.blocks_per_cu < 0) throw std::invalid_argument("blocks_per_cu must be >= 0");
+  if (o.lu_layout != 0 && o.lu_layout != 1) throw std::invalid_argument("lu_layout must be 0 or 1");
 }
 
 // Launches the persistent BDF kernel on arrays already resident on the device.
@@ -123,7 +124,8 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   const int block = 64 * wpb;
   const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) +
                      static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
-  const void* fn = hpo::bdf_kernel_fn(ns, o.lu_precision, d_prof != nullptr);
+  const int variant = hpo::bdf_variant(o.lu_precision, o.lu_layout);
+  const void* fn = hpo::bdf_kernel_fn(ns, variant, d_prof != nullptr);
   int occ = 0;
   HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
   if (occ < 1) throw std::invalid_argument("kernel does not fit on a CU with this configuration");
@@ -166,7 +168,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
 
   HPO_HIP(hipMemsetAsync(res->d_queue, 0, sizeof(uint32_t), st));
   if (tm) HPO_HIP(hipEventRecord(m->ev0, st));
-  if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, o.lu_precision, static_cast<int>(grid), block, lds, st));
+  if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
   if (tm) HPO_HIP(hipEventRecord(m->ev1, st));
   if (tm) {
     HPO_HIP(hipEventSynchronize(m->ev1));
@@ -187,11 +189,11 @@ int64_t max_wave_slots(hpo_mech* m) {
   // one-wave workgroups pack the most waves onto a CU; the launch grid never exceeds
   // n_CU x (workgroups per CU) x (waves per workgroup)
   for (int wpb = 1; wpb <= 4; ++wpb)
-    for (int lu = 0; lu < 2; ++lu) {
+    for (int variant = 0; variant < 3; ++variant) {
       const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) +
                          static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
       int occ = 0;
-      HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_kernel_fn(ns, lu), 64 * wpb, lds));
+      HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_kernel_fn(ns, variant), 64 * wpb, lds));
       most = std::max<int64_t>(most, static_cast<int64_t>(occ) * wpb);
     }
   return static_cast<int64_t>(m->n_cu) * most;
@@ -616,11 +618,31 @@ int hpo_autotune(const hpo_mech* mc, int64_t ncells, const double* d_y, int64_t 
         R.waves_per_block[k] = wpbs[wi];
         R.blocks_per_cu[k] = (t.grid > 0 && m->n_cu > 0) ? std::max(1, t.grid / m->n_cu) : 0;
         R.order_mode[k] = om;
+        R.lu_layout[k] = c.lu_layout;
         R.sample_ms[k] = t.kernel_ms;
         if (t.kernel_ms < best_ms) best_ms = t.kernel_ms, best = k;
       }
+    // FP32 path: the other Newton-matrix layout (explicit inverse / in-place LU) at the best
+    // launch shape and order
+    if (o->lu_precision == 1 && best >= 0 && R.n_candidates < HPO_TUNE_MAX) {
+      const int om = R.order_mode[best];
+      HPO_HIP(hipMemcpyAsync(sorder, sample_orders[om].data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+      hpo_opts c = base;
+      c.waves_per_block = R.waves_per_block[best];
+      c.lu_layout = 1 - base.lu_layout;
+      hpo_timing t{};
+      launch_integrate(m, S, sy, syo, ns, sP, sdt, sst, nullptr, sorder, nullptr, c, st, &t);
+      const int k = R.n_candidates++;
+      R.waves_per_block[k] = c.waves_per_block;
+      R.blocks_per_cu[k] = (t.grid > 0 && m->n_cu > 0) ? std::max(1, t.grid / m->n_cu) : 0;
+      R.order_mode[k] = om;
+      R.lu_layout[k] = c.lu_layout;
+      R.sample_ms[k] = t.kernel_ms;
+      if (t.kernel_ms < best_ms) best_ms = t.kernel_ms, best = k;
+    }
     R.best = best;
     o->waves_per_block = R.waves_per_block[best];
+    o->lu_layout = R.lu_layout[best];
     o->blocks_per_cu = 0;   // as many workgroups per CU as fit (what every candidate ran with)
     o->sort_cells = 0;
 

@@ -56,7 +56,9 @@ typedef struct {
   int32_t blocks_per_cu;   /* persistent workgroups per CU; 0 = as many as fit */
   int32_t sort_cells;      /* work-queue order: 0 cell index; 1 descending initial temperature;
                               2 (hpo_batch_integrate only) the order of the last hpo_batch_autotune */
-  int32_t reserved[2];
+  int32_t lu_layout;       /* with lu_precision == 1: 0 explicit inverse (Gauss-Jordan, solves as
+                              matrix-vector products); 1 in-place LU (triangular solves) */
+  int32_t reserved;
 } hpo_opts;
 
 typedef struct {
@@ -125,6 +127,7 @@ int hpo_integrate_device(const hpo_mech *m, int64_t ncells, const double *d_y_in
 typedef struct {
   int32_t n_candidates, best;
   int32_t waves_per_block[HPO_TUNE_MAX], blocks_per_cu[HPO_TUNE_MAX], order_mode[HPO_TUNE_MAX];
+  int32_t lu_layout[HPO_TUNE_MAX];
   double sample_ms[HPO_TUNE_MAX];
   int64_t sample_cells;
   double tune_ms;          /* wall time of the whole tuning call */

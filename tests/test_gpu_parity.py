@@ -158,11 +158,12 @@ def test_integrate_h2_matches_oracle(h2):
     _compare_states(yg, yo, 1e-5, 100.0)
 
 
-@pytest.mark.parametrize("sweeps", [0, 1, 2])
-def test_integrate_gri_fp32_lu(gri, sweeps):
+@pytest.mark.parametrize("sweeps,layout", [(0, 0), (1, 0), (2, 0), (1, 1)])
+def test_integrate_gri_fp32_lu(gri, sweeps, layout):
+    """FP32 Newton matrix (layout 0: explicit inverse, 1: in-place LU) with FP64 refinement."""
     m, o = gri
     y0, P, dt = WL.gri_plain(m.species, m.W, 64, seed=13)
-    yg, sg, stg = m.integrate(y0, P, dt, lu_precision=1, refine_sweeps=sweeps)
+    yg, sg, stg = m.integrate(y0, P, dt, lu_precision=1, refine_sweeps=sweeps, lu_layout=layout)
     yo, so, sto = o.integrate(P, dt, y0, nthreads=8)
     assert (sg == 0).all()
     _compare_states(yg, yo, 1e-5, 300.0)
