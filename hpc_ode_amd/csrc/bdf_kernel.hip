@@ -455,6 +455,11 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         // cvLsSetup / cvLsLinSys
         const double gamma = S.gamma;
         const int ol = opaque(lane);
+        // J's columns are packed at the state length: lanes >= NS read entries of other rows
+        // (or never-written scratch, still inside the 64 x 64 J block). Their Newton-matrix rows
+        // are never pivot candidates and are zeroed when the factors are permuted, and the
+        // refinement's products on those lanes are discarded, so nothing they hold reaches the
+        // live lanes.
         const double* Jrow = g.J + ol;
         double dgamma = fabs(gamma / S.gammap - 1.0);
         bool jnew = (nst == 0) || (nst >= nstlj + MSBJ) || (convfail == FAIL_BAD_J && dgamma < DGMAX_JBAD) ||
