@@ -209,3 +209,26 @@ def test_rhs_and_jacobian_h2li_xml_mechanism(h2li_path):
     ref = np.stack([o.jac(P[i], y[i]) for i in range(len(y))])
     scale = np.abs(ref).max(axis=(1, 2), keepdims=True)
     assert (np.abs(J - ref) / scale).max() < 1e-10
+
+
+@pytest.mark.parametrize("lu_precision", [0, 1])
+def test_failure_codes_match_oracle(gri, lu_precision):
+    """Per-cell CVODE return codes (the reference ignores them, CVodeIntegrator.cpp:143; the
+    build reports them): a step budget that runs out (CV_TOO_MUCH_WORK) beside a zero interval
+    (CV_TOO_CLOSE) in the same launch, and a zero error tolerance on zero components
+    (CV_ILL_INPUT), against the oracle's codes (and step counts) on the same inputs."""
+    m, o = gri
+    y0, P, dt = WL.gri_plain(m.species, m.W, 8, seed=21)
+    dt = dt.copy()
+    dt[3] = 0.0                                        # t0 == tout
+    # step budget: 5 steps are not enough for any of these cells
+    yg, sg, cg = m.integrate(y0, P, dt, maxsteps=5, lu_precision=lu_precision)
+    yo, so, co = o.integrate(P, dt, y0, maxsteps=5, nthreads=8)
+    assert so.tolist() == [-1, -1, -1, -27, -1, -1, -1, -1]
+    assert sg.tolist() == so.tolist()
+    assert np.array_equal(cg["nst"][so == -1], co["nst"][so == -1])
+    # atol = 0 with zero mass fractions (every cell starts without atomic C): the error
+    # weights are undefined
+    yg, sg, _ = m.integrate(y0, P, np.full(8, 1e-6), atol=0.0, lu_precision=lu_precision)
+    yo, so, _ = o.integrate(P, np.full(8, 1e-6), y0, atol=0.0, nthreads=8)
+    assert so.tolist() == [-22] * 8 and sg.tolist() == so.tolist()
