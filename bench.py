@@ -120,6 +120,21 @@ def _errors(y, truth):
             "Y_max_rel_above_1e-6": float(rel[np.abs(Yt) > 1e-6].max())}
 
 
+def _agreement(y, ref, rtol=1e-6, atol=1e-10):
+    """GPU against the CPU BDF at the same tolerances, cell by cell: the north star's match
+    criterion (relative rtol on T and on every mass fraction above 1e-10) and CVODE's own
+    per-component weight (|dy| <= rtol |y| + atol)."""
+    dT = np.abs(y[:, 0] - ref[:, 0]) / ref[:, 0]
+    Yr, dY = ref[:, 1:], np.abs(y[:, 1:] - ref[:, 1:])
+    strict = (dT <= rtol) & np.all((np.abs(Yr) <= 1e-10) | (dY <= rtol * np.abs(Yr)), axis=1)
+    w = dY / (rtol * np.abs(Yr) + atol)
+    weighted = (dT <= rtol) & np.all(w <= 1.0, axis=1)
+    return {"T_max_rel": float(dT.max()), "T_median_rel": float(np.median(dT)),
+            "cells_within_rtol_T_and_Y_above_1e-10": float(strict.mean()),
+            "cells_within_rtol_y_plus_atol": float(weighted.mean()),
+            "max_dy_over_rtol_y_plus_atol": float(w.max())}
+
+
 def accuracy_statement(orc, y0, P, dt, y32, y64, ncells):
     """Errors of the GPU results on an evenly spaced subset of the workload against the same
     BDF run on the CPU at rtol 1e-12 / atol 1e-16, beside the CPU run at the production
@@ -136,8 +151,11 @@ def accuracy_statement(orc, y0, P, dt, y32, y64, ncells):
            "truth_failed": int((~ok).sum()), "cpu_seconds": took,
            "fp32lu_gpu": _errors(y32[idx][ok], truth[ok]),
            "cpu_rtol1e-6": _errors(y6[ok], truth[ok])}
+    ok6 = st6 == 0
+    out["fp32lu_gpu_vs_cpu_rtol1e-6"] = _agreement(y32[idx][ok6], y6[ok6])
     if y64 is not None:
         out["fp64_gpu"] = _errors(y64[idx][ok], truth[ok])
+        out["fp64_gpu_vs_cpu_rtol1e-6"] = _agreement(y64[idx][ok6], y6[ok6])
     return out
 
 
