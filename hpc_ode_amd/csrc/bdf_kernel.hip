@@ -15,17 +15,18 @@
 
 // Occupancy targets of the BDF kernel (waves per SIMD). At 1 a wave may use the whole
 // 512-entry unified VGPR/AGPR file: the FP64 path keeps its 106-register Newton-matrix row,
-// the Nordsieck history and the RHS working set there without scratch spills. The FP32-LU
-// path holds a 53-register row; at 2 waves per SIMD it still spills (the RHS and solve
-// working sets peak together), so both default to 1.
+// the Nordsieck history and the RHS working set there without scratch spills. The FP32
+// path holds a 53-register row and fits 256 registers, so it runs two waves per SIMD (a few
+// dozen bytes per lane spill, where the RHS and solve working sets peak together).
 #ifndef HPO_WAVES_PER_EU_FP64
 #define HPO_WAVES_PER_EU_FP64 1
 #endif
-// Loads of the refinement's FP64 Jacobian row issued together. The sweep is bound by the
-// bandwidth of these (cache-missing) loads, not their latency: groups of 8 measured fastest,
-// larger groups only add registers in flight (27: +20 % kernel time, 53: +35 %).
-#ifndef HPO_JX_UNROLL
-#define HPO_JX_UNROLL 8
+// Most loads of the refinement's FP64 Jacobian row issued together. Each group costs one
+// round trip to the (cache-missing) Jacobian, so fewer, larger groups wait less, at two
+// registers per load in flight. GRI's 53 columns as two groups of 27 measured fastest on
+// 262,144 cells (1,262 ms against 1,292 ms for seven groups of 8); one group of 53 spills.
+#ifndef HPO_JX_GROUP
+#define HPO_JX_GROUP 27
 #endif
 #ifndef HPO_WAVES_PER_EU_FP32
 #define HPO_WAVES_PER_EU_FP32 2
@@ -33,6 +34,23 @@
 
 namespace hpo {
 namespace dev {
+
+// The refinement's J x product reads its NS columns in jx_groups() equal groups of
+// jx_group() loads, NS rounded up to jx_cols(). The padding columns lie inside the wave's
+// 64 x 64 Jacobian block, after the NS x NS entries the Jacobian writes, and are zeroed at
+// kernel entry.
+template <int NS>
+constexpr int jx_groups() {
+  return (NS + HPO_JX_GROUP - 1) / HPO_JX_GROUP;
+}
+template <int NS>
+constexpr int jx_group() {
+  return (NS + jx_groups<NS>() - 1) / jx_groups<NS>();
+}
+template <int NS>
+constexpr int jx_cols() {
+  return jx_groups<NS>() * jx_group<NS>();
+}
 
 // CVODE constants (SUNDIALS 6.x defaults)
 constexpr int QMAX = 5;
@@ -525,9 +543,19 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
             wave_sync();
             w.hRT[lane] = x;
             wave_sync();
+            // Whole groups of JG loads are issued before their products, one round trip per
+            // group. Columns NS..JN-1 are zero padding (set at kernel entry) and x is zero on
+            // lanes >= NS, so the padded products add nothing.
+            constexpr int JG = jx_group<NS>();
             double Jx = 0.0;
-#pragma unroll HPO_JX_UNROLL
-            for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j * NS], w.hRT[j], Jx);
+#pragma unroll 1
+            for (int j0 = 0; j0 < jx_cols<NS>(); j0 += JG) {
+              double v[JG];
+#pragma unroll
+              for (int k = 0; k < JG; ++k) v[k] = Jrow[(j0 + k) * NS];
+#pragma unroll
+              for (int k = 0; k < JG; ++k) Jx = fma(v[k], w.hRT[j0 + k], Jx);
+            }
             double r = b - x + gl * Jx;
             float d32 = solve32<NS, INV>(lu.L, static_cast<float>(r), lane, vbuf);
             x += static_cast<double>(d32);
@@ -716,6 +744,9 @@ __global__ __launch_bounds__(256) __attribute__((
   const int64_t slot = static_cast<int64_t>(blockIdx.x) * wpb + wave;
   WaveScratch g = make_wave_scratch(A.scratch + slot * A.scratch_stride, Lay);
   if (lane < 16) g.coefbuf[lane] = t.coef[lane];
+  static_assert((jx_cols<NS>() - 1) * NS + 64 <= 64 * 64, "padded J x columns leave the Jacobian block");
+  for (int i = NS * NS + lane; i < (jx_cols<NS>() - 1) * NS + 64; i += 64) g.J[i] = 0.0;
+  wave_sync();
   for (;;) {
     unsigned int c = 0;
     if (lane == 0) c = atomicAdd(A.queue, 1u);
