@@ -149,11 +149,26 @@ def test_integrate_gri_matches_oracle(gri):
     assert (stg["nst"] == sto["nst"]).mean() > 0.3
 
 
-def test_integrate_h2_matches_oracle(h2):
+@pytest.mark.parametrize("lu_precision,sweeps", [(0, 1), (1, 1), (1, 2)])
+def test_integrate_h2_matches_oracle(h2, lu_precision, sweeps):
+    """The 10-variable kernel, FP64 LU and FP32 inverse with one or two refinement sweeps
+    (the refinement's J x reads its 10 columns as one group, no padding)."""
     m, o = h2
     y0, P, dt = WL.h2_air(m.species, m.W, 100, seed=12)
-    yg, sg, stg = m.integrate(y0, P, dt)
+    yg, sg, stg = m.integrate(y0, P, dt, lu_precision=lu_precision, refine_sweeps=sweeps)
     yo, so, sto = o.integrate(P, dt, y0, nthreads=8)
+    assert (sg == 0).all() and (so == 0).all()
+    _compare_states(yg, yo, 1e-5, 100.0)
+
+
+@pytest.mark.parametrize("lu_precision", [0, 1])
+def test_integrate_h2li_xml_mechanism(h2li_path, lu_precision):
+    """The 9-variable kernel (h2_li_2004.xml) integrated against the oracle in both precisions."""
+    import oracle as O
+    m, o = H.Mechanism(h2li_path), O.Oracle(h2li_path)
+    y0, P, dt = WL.h2_air(m.species, m.W, 64, seed=43)
+    yg, sg, _ = m.integrate(y0, P, dt, lu_precision=lu_precision)
+    yo, so, _ = o.integrate(P, dt, y0, nthreads=8)
     assert (sg == 0).all() and (so == 0).all()
     _compare_states(yg, yo, 1e-5, 100.0)
 
