@@ -215,29 +215,38 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         if (j <= q) zn[j - 1] += zn[j];
     {
       // cvSetBDF
+      // The l polynomial is built in registers (loops unrolled over the order, guarded by
+      // the wave-uniform q) and stored once: in LDS every update would be a dependent
+      // load-store round trip.
       const double h = S.h;
       double alpha0 = -1.0, alpha0_hat = -1.0, xi_inv = 1.0, xistar_inv = 1.0, hsum = h;
-      S.l[0] = 1.0;
-      S.l[1] = 1.0;
-      for (int i = 2; i <= QMAX; ++i) S.l[i] = 0.0;
+      double l[QMAX + 1] = {1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
       if (q > 1) {
-        for (int j = 2; j < q; ++j) {
-          hsum += S.tau[j - 1];
-          xi_inv = h / hsum;
-          alpha0 -= 1.0 / j;
-          for (int i = j; i >= 1; --i) S.l[i] += S.l[i - 1] * xi_inv;
+#pragma unroll
+        for (int j = 2; j < QMAX; ++j) {
+          if (j < q) {
+            hsum += S.tau[j - 1];
+            xi_inv = h / hsum;
+            alpha0 -= 1.0 / j;
+#pragma unroll
+            for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+          }
         }
         alpha0 -= 1.0 / q;
-        xistar_inv = -S.l[1] - alpha0;
+        xistar_inv = -l[1] - alpha0;
         hsum += S.tau[q - 1];
         xi_inv = h / hsum;
-        alpha0_hat = -S.l[1] - xi_inv;
-        for (int i = q; i >= 1; --i) S.l[i] += S.l[i - 1] * xistar_inv;
+        alpha0_hat = -l[1] - xi_inv;
+#pragma unroll
+        for (int i = QMAX; i >= 1; --i)
+          if (i <= q) l[i] += l[i - 1] * xistar_inv;
       }
+#pragma unroll
+      for (int i = 0; i <= QMAX; ++i) S.l[i] = l[i];
       // cvSetTqBDF
       double A1 = 1.0 - alpha0_hat + alpha0;
       double A2 = 1.0 + q * A1;
-      double lq = S.l[q];
+      double lq = pick(l, q);
       S.tq[2] = fabs(A1 / (alpha0 * A2));
       S.tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
       if (qwait == 1) {
@@ -258,7 +267,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         S.tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
       }
       S.tq[4] = NLSCOEF / S.tq[2];
-      S.rl1 = 1.0 / S.l[1];
+      S.rl1 = 1.0 / l[1];
       S.gamma = h * S.rl1;
       if (nst == 0) S.gammap = S.gamma;
       S.gamrat = (nst > 0) ? S.gamma / S.gammap : 1.0;
@@ -367,9 +376,17 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
     nst++;
     const double h = S.h;
     S.hu = h;
-    for (int i = q; i >= 2; --i) S.tau[i] = S.tau[i - 1];
-    if (q == 1 && nst > 1) S.tau[2] = S.tau[1];
-    S.tau[1] = h;
+    {
+      // shift the step-size history: all loads before the stores, not one round trip each
+      double tau[QMAX];
+#pragma unroll
+      for (int i = 1; i < QMAX; ++i) tau[i] = S.tau[i];
+#pragma unroll
+      for (int i = QMAX; i >= 2; --i)
+        if (i <= q) S.tau[i] = tau[i - 1];
+      if (q == 1 && nst > 1) S.tau[2] = tau[1];
+      S.tau[1] = h;
+    }
 #pragma unroll
     for (int j = 0; j <= QMAX; ++j)
       if (j <= q) zn[j] += S.l[j] * acor;
