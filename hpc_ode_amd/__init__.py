@@ -112,6 +112,8 @@ def lib():
         L.hpo_eval_rhs.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
         L.hpo_eval_jac.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
         L.hpo_lu_solve.argtypes = [C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip]
+        L.hpo_lu_solve_layout.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip,
+                                          C.POINTER(C.c_float)]
         L.hpo_device_info.argtypes = [_ip, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.hpo_autotune.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp, C.c_int64, C.POINTER(Opts), vp,
                                    C.POINTER(TuneReport), vp]
@@ -336,15 +338,22 @@ class Server:
         return u.value, b.value
 
 
-def lu_solve(M, b, fp32=False):
-    """Batched LU + solve with the kernel's register LU (parity tests)."""
+def lu_solve(M, b, fp32=False, layout=None, timing=False):
+    """Batched LU + solve (parity tests). layout None: hpo_lu_solve, the BDF kernel's register
+    code; 0 / 1: hpo_lu_solve_layout with one wave / one 256-thread workgroup per matrix.
+    timing=True also returns the launch's device time in ms (layouts 0 and 1)."""
     M = np.ascontiguousarray(M, np.float64)
     b = np.ascontiguousarray(b, np.float64)
     n, ns, _ = M.shape
     x = np.zeros((n, ns))
     ok = np.zeros(n, np.int32)
-    _check(lib().hpo_lu_solve(ns, int(fp32), n, _p(M), _p(b), _p(x), _p(ok, _ip)))
-    return x, ok
+    if layout is None:
+        _check(lib().hpo_lu_solve(ns, int(fp32), n, _p(M), _p(b), _p(x), _p(ok, _ip)))
+        return x, ok
+    ms = C.c_float()
+    _check(lib().hpo_lu_solve_layout(ns, int(fp32), int(layout), n, _p(M), _p(b), _p(x), _p(ok, _ip),
+                                     C.byref(ms)))
+    return (x, ok, ms.value) if timing else (x, ok)
 
 
 def device_info():

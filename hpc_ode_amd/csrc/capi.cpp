@@ -488,6 +488,49 @@ int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double* M, const dou
   return rc;
 }
 
+int hpo_lu_solve_layout(int32_t ns, int32_t fp32, int32_t layout, int64_t n, const double* M, const double* b,
+                        double* x, int32_t* ok, float* kernel_ms) {
+  if (!M || !b || !x || !ok || n < 0) return fail(HPO_E_INVALID, "bad argument");
+  if (layout != 0 && layout != 1) return fail(HPO_E_INVALID, "layout must be 0 (wave per matrix) or 1 (workgroup per matrix)");
+  if (layout == 1 && fp32) return fail(HPO_E_UNSUPPORTED, "the workgroup-per-matrix layout is FP64 only");
+  if (!hpo::kernel_supports(ns)) return fail(HPO_E_UNSUPPORTED, "no kernel for this size");
+  if (n > 0x7fffffffLL) return fail(HPO_E_INVALID, "too many matrices for one launch");
+  if (kernel_ms) *kernel_ms = 0.f;
+  if (n == 0) return HPO_OK;
+  double *dM = nullptr, *db = nullptr, *dx = nullptr;
+  int32_t* dok = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = HPO_OK;
+  try {
+    dM = dalloc<double>(static_cast<size_t>(n * ns * ns));
+    db = dalloc<double>(static_cast<size_t>(n * ns));
+    dx = dalloc<double>(static_cast<size_t>(n * ns));
+    dok = dalloc<int32_t>(n);
+    HPO_HIP(hipMemcpy(dM, M, n * ns * ns * sizeof(double), hipMemcpyHostToDevice));
+    HPO_HIP(hipMemcpy(db, b, n * ns * sizeof(double), hipMemcpyHostToDevice));
+    HPO_HIP(hipEventCreate(&e0));
+    HPO_HIP(hipEventCreate(&e1));
+    hpo::LuArgs A{dM, db, dx, dok};
+    HPO_HIP(hipEventRecord(e0, nullptr));
+    if (layout == 0)
+      HPO_HIP(hpo::launch_lu(ns, fp32, A, n, nullptr));
+    else
+      HPO_HIP(hpo::launch_lu_block(ns, A, n, nullptr));
+    HPO_HIP(hipEventRecord(e1, nullptr));
+    HPO_HIP(hipEventSynchronize(e1));
+    if (kernel_ms) HPO_HIP(hipEventElapsedTime(kernel_ms, e0, e1));
+    HPO_HIP(hipMemcpy(x, dx, n * ns * sizeof(double), hipMemcpyDeviceToHost));
+    HPO_HIP(hipMemcpy(ok, dok, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } catch (...) {
+    rc = map_exceptions();
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (void* p : {static_cast<void*>(dM), static_cast<void*>(db), static_cast<void*>(dx), static_cast<void*>(dok)})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
 namespace {
 
 // Mean step count per (T0, log10 dt) bin of the calibration sample; bins the sample left

@@ -71,5 +71,7 @@ hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block
 inline int bdf_variant(int lu_precision, int lu_layout) { return lu_precision == 0 ? 0 : (lu_layout == 0 ? 1 : 2); }
 hipError_t launch_eval(int ns, const EvalArgs& A, int grid, int block, size_t shmem, hipStream_t st);
 hipError_t launch_lu(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st);
+// one 256-thread workgroup per matrix, FP64 (lu_block.hip)
+hipError_t launch_lu_block(int ns, const LuArgs& A, int64_t ncases, hipStream_t st);
 
 }  // namespace hpo

@@ -201,6 +201,14 @@ int hpo_eval_jac(const hpo_mech *m, int64_t n, const double *y, const double *P,
  * FP32 Gauss-Jordan inverse (same pivots) applied as a matrix-vector product, no refinement. */
 int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double *M, const double *b, double *x,
                  int32_t *ok);
+/* hpo_lu_solve with the batched layout chosen, in place of the reference GPU path's MAGMA
+ * batched LU (magma_dgetrf_batched / magma_dgetrs_batched via SUNLinSol_MagmaDense,
+ * src/Integrators/CVodeIntegratorGPU.cpp:86-91): layout 0 = one wave per matrix (the
+ * register code above, fp32 0 or 1); layout 1 = one 256-thread workgroup per matrix, the
+ * augmented matrix [M | b] in LDS, FP64 only (fp32 = 1 returns HPO_E_UNSUPPORTED).
+ * kernel_ms (may be NULL): the launch's device time from HIP events. */
+int hpo_lu_solve_layout(int32_t ns, int32_t fp32, int32_t layout, int64_t n, const double *M,
+                        const double *b, double *x, int32_t *ok, float *kernel_ms);
 
 const char *hpo_last_error(void);
 /* Device properties used for sizing: CU count, free/total HBM bytes. */

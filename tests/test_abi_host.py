@@ -65,3 +65,12 @@ def test_invalid_arguments_return_error_codes():
     L.hpo_opts_default(C.byref(o))
     assert o.rtol == 1e-6 and o.atol == 1e-10 and o.maxsteps == 10000
     assert L.hpo_lu_solve(7, 0, 1, None, None, None, None) == -1
+    # the layout checks come before any device call (HPO_E_INVALID = -1, HPO_E_UNSUPPORTED = -5)
+    M, b, x = np.eye(10)[None], np.ones((1, 10)), np.zeros((1, 10))
+    ok = np.zeros(1, np.int32)
+    args = [M.ctypes.data_as(C.POINTER(C.c_double)), b.ctypes.data_as(C.POINTER(C.c_double)),
+            x.ctypes.data_as(C.POINTER(C.c_double)), ok.ctypes.data_as(C.POINTER(C.c_int32)), None]
+    assert L.hpo_lu_solve_layout(10, 0, 1, 1, None, None, None, None, None) == -1
+    assert L.hpo_lu_solve_layout(10, 0, 2, 1, *args) == -1
+    assert L.hpo_lu_solve_layout(10, 1, 1, 1, *args) == -5
+    assert L.hpo_lu_solve_layout(7, 0, 1, 1, *args) == -5

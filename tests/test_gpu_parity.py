@@ -121,6 +121,49 @@ def test_lu_random_dense_pivoting(fp32):
             assert err <= 1e-9 * np.linalg.cond(M[i]) ** 0.5, (i, err)
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+def test_lu_layouts_newton_and_dense(gri, h2, layout):
+    """hpo_lu_solve_layout, FP64: one wave per matrix (0) and one workgroup per matrix (1), on
+    GRI and H2 Newton matrices and on random dense matrices that need row exchanges, against
+    LAPACK and against the oracle's SUNDIALS-order getrf/getrs; a singular matrix is reported."""
+    import oracle as O
+    rng = np.random.default_rng(17)
+    cases = []
+    for mech, states in ((gri, lambda m: _gri_states(m, 12, seed=15)),
+                         (h2, lambda m: WL.h2_air(m.species, m.W, 12, seed=16)[:2])):
+        m, _ = mech
+        y, P = states(m)
+        gam = 10.0 ** rng.uniform(-9, -6, len(y))
+        cases.append((np.eye(m.n_state)[None] - gam[:, None, None] * m.jac(y, P), []))
+    D = rng.standard_normal((6, 53, 53))
+    D[2] = np.triu(D[2])[::-1]           # a pivot in every column
+    D[4, :, 11] = 0.0                    # singular
+    cases.append((D, [4]))
+    for M, singular in cases:
+        n, ns, _ = M.shape
+        b = rng.standard_normal((n, ns))
+        x, ok, ms = H.lu_solve(M, b, layout=layout, timing=True)
+        assert ms > 0.0
+        sing = np.isin(np.arange(n), singular)
+        assert ok.tolist() == (~sing).astype(int).tolist()
+        for i in np.flatnonzero(~sing):
+            ref = np.linalg.solve(M[i], b[i])
+            err = np.abs(x[i] - ref).max() / np.abs(ref).max()
+            assert err <= 50 * np.linalg.cond(M[i]) * np.finfo(np.float64).eps, (ns, i, err)
+            lu, piv, ier = O.getrf(M[i])
+            assert ier == 0
+            xo = O.getrs(lu, piv, b[i])
+            # the same operations in the same order; only fused multiply-adds round differently
+            diff = np.abs(x[i] - xo).max() / np.abs(xo).max()
+            assert diff <= 10 * np.linalg.cond(M[i]) * np.finfo(np.float64).eps, (ns, i, diff)
+
+
+def test_lu_block_layout_rejects_fp32():
+    M, b = np.eye(10)[None], np.ones((1, 10))
+    with pytest.raises(H.HpoError):
+        H.lu_solve(M, b, fp32=True, layout=1)
+
+
 def _compare_states(yg, yo, T_rtol, w_max, rtol=1e-6, atol=1e-10):
     """GPU vs oracle on the same inputs. T: relative difference below T_rtol. All state
     components: |dy| / (rtol |y| + atol) below w_max. Both integrations run the same BDF
