@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--tune-sample", type=int, default=32768)
     ap.add_argument("--serve-seconds", type=float, default=16.0,
                     help="duration of the serving-loop measurement, split over the offered loads (0 = skip)")
+    ap.add_argument("--cu-share", type=int, default=0,
+                    help="serving: largest share of the device one launch may take, in percent (0 = 100)")
     ap.add_argument("--serve-loads", default="0.3,0.7",
                     help="offered loads as fractions of the measured serving capacity")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -75,14 +77,14 @@ def measured_traffic(workload, ncells, lu_precision):
     return rec.get("traffic_bytes")
 
 
-def serving_measurement(H, gri, seconds, fractions):
+def serving_measurement(H, gri, seconds, fractions, cu_share=0):
     """p50/p99 substep latency of the serving loop at offered loads stated as fractions of its
     measured capacity (requests: 70 % H2/air, 30 % GRI-3.0, log-uniform [1, 65536] cells,
     dt U[1e-5, 1e-4] s, two substeps per request stream, Poisson arrivals)."""
     from hpc_ode_amd import serving as SV
     h2 = H.Mechanism(os.path.join(ROOT, "hpc_ode_amd", "mechanisms", "h2o2.yaml"))
     mechs = [h2, gri]
-    srv = H.Server(mechs, n_slots=4)
+    srv = H.Server(mechs, n_slots=4, cu_share_pct=cu_share)
     try:
         cap = SV.capacity(srv, mechs)
         mc = SV.mean_cells()
@@ -95,6 +97,7 @@ def serving_measurement(H, gri, seconds, fractions):
             r["fraction_of_capacity"] = f
             loads.append(r)
         return {"capacity_cells_per_s": cap, "mean_cells_per_request": mc, "slots": 4,
+                "cu_share_pct": cu_share or 100,
                 "mix": "70% H2/air (h2o2.yaml), 30% GRI-3.0; cells log-U[1,65536]; dt U[1e-5,1e-4] s; "
                        "2 substeps per stream with state resident between them; Poisson arrivals",
                 "loads": loads}
@@ -301,7 +304,8 @@ def main():
 
     serving = None
     if a.serve_seconds > 0:
-        serving = serving_measurement(H, mech, a.serve_seconds, [float(x) for x in a.serve_loads.split(",")])
+        serving = serving_measurement(H, mech, a.serve_seconds, [float(x) for x in a.serve_loads.split(",")],
+                                      a.cu_share)
 
     value = n / (ms32 / 1e3)
     roof = None

@@ -64,7 +64,7 @@ class TuneReport(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("n_slots", C.c_int32), ("reserved0", C.c_int32), ("max_batch_cells", C.c_int64),
+    _fields_ = [("n_slots", C.c_int32), ("cu_share_pct", C.c_int32), ("max_batch_cells", C.c_int64),
                 ("hbm_budget_bytes", C.c_int64), ("opts", Opts)]
 
 
@@ -273,10 +273,11 @@ class Batch:
 class Server:
     """Serving loop (hpo_server): resident cell streams, substep requests, continuous batching."""
 
-    def __init__(self, mechs, n_slots=4, max_batch_cells=262144, hbm_budget_bytes=0, **opt_kw):
+    def __init__(self, mechs, n_slots=4, max_batch_cells=262144, hbm_budget_bytes=0, cu_share_pct=0, **opt_kw):
         so = ServerOpts()
         lib().hpo_server_opts_default(C.byref(so))
         so.n_slots, so.max_batch_cells, so.hbm_budget_bytes = n_slots, max_batch_cells, hbm_budget_bytes
+        so.cu_share_pct = cu_share_pct
         so.opts = opts(**opt_kw)
         self.mechs = list(mechs)
         arr = (C.c_void_p * len(self.mechs))(*[m.handle.value for m in self.mechs])

@@ -131,6 +131,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   if (occ < 1) throw std::invalid_argument("kernel does not fit on a CU with this configuration");
   int bpc = o.blocks_per_cu > 0 ? std::min(o.blocks_per_cu, occ) : occ;
   int64_t grid = static_cast<int64_t>(m->n_cu) * bpc;
+  if (res && res->share_pct < 100) grid = (grid * std::max(res->share_pct, 1) + 99) / 100;
   grid = std::min<int64_t>(grid, (ncells + wpb - 1) / wpb);
   grid = std::max<int64_t>(grid, 1);
   ExecRes own;

@@ -44,6 +44,7 @@ struct ExecRes {
   double* d_scratch = nullptr;
   int64_t scratch_slots = 0;    // wave slots the scratch holds
   int64_t scratch_stride = 0;   // doubles per wave slot
+  int32_t share_pct = 100;      // largest share of n_CU x workgroups-per-CU the grid may take
 };
 
 void check_opts(const hpo_opts& o);

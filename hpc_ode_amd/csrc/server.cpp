@@ -259,6 +259,9 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
     if (s->opts.n_slots <= 0) s->opts.n_slots = 4;
     if (s->opts.n_slots > 16) throw std::invalid_argument("n_slots must be at most 16");
     if (s->opts.max_batch_cells <= 0) s->opts.max_batch_cells = 262144;
+    if (s->opts.cu_share_pct == 0) s->opts.cu_share_pct = 100;
+    if (s->opts.cu_share_pct < 1 || s->opts.cu_share_pct > 100)
+      throw std::invalid_argument("cu_share_pct must be in 0..100");
     check_opts(s->opts.opts);
     if (s->opts.opts.sort_cells == 2) throw std::invalid_argument("sort_cells = 2 needs a batch");
     HPO_HIP(hipGetDevice(&s->device));
@@ -284,6 +287,7 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
       sl.res.d_queue = dalloc<uint32_t>(4);
       sl.res.scratch_stride = stride;
       sl.res.scratch_slots = wslots;
+      sl.res.share_pct = s->opts.cu_share_pct;
       sl.res.d_scratch = dalloc<double>(static_cast<size_t>(stride * wslots));
       sl.d_y = dalloc<double>(static_cast<size_t>(sl.cap * ldmax));
       sl.d_P = dalloc<double>(sl.cap);

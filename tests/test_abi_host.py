@@ -74,3 +74,20 @@ def test_invalid_arguments_return_error_codes():
     assert L.hpo_lu_solve_layout(10, 0, 2, 1, *args) == -1
     assert L.hpo_lu_solve_layout(10, 1, 1, 1, *args) == -5
     assert L.hpo_lu_solve_layout(7, 0, 1, 1, *args) == -5
+
+
+def test_server_option_checks_come_before_device_calls(h2_path):
+    """hpo_server_create rejects a CU share outside 0..100 and too many slots before it touches
+    the device."""
+    m = H.Mechanism(h2_path)
+    L = H.lib()
+    arr = (C.c_void_p * 1)(m.handle.value)
+    for field, bad in (("cu_share_pct", 101), ("cu_share_pct", -5), ("n_slots", 17)):
+        so = H.ServerOpts()
+        L.hpo_server_opts_default(C.byref(so))
+        assert so.cu_share_pct == 0 and so.n_slots == 4
+        setattr(so, field, bad)
+        h = C.c_void_p()
+        assert L.hpo_server_create(arr, 1, C.byref(so), C.byref(h)) == -1
+        assert not h.value
+        assert field.encode() in L.hpo_last_error()

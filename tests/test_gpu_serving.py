@@ -23,8 +23,11 @@ def _state(m, which, n, seed):
     return SV.initial_states(m, which, n, seed)
 
 
-def test_result_same_alone_and_in_mix(mechs):
-    srv = H.Server(mechs, n_slots=3)
+@pytest.mark.parametrize("cu_share", [0, 30])
+def test_result_same_alone_and_in_mix(mechs, cu_share):
+    """With cu_share 30 every launch takes at most 30 % of the device's workgroup slots, so
+    the mix runs as concurrent launches of smaller grids."""
+    srv = H.Server(mechs, n_slots=3, cu_share_pct=cu_share)
     try:
         gri = mechs[1]
         y0, P = _state(gri, 1, 100, 5)
