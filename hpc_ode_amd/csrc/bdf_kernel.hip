@@ -886,6 +886,19 @@ const void* bdf_kernel_fn(int ns, int variant, int prof) {
   return nullptr;
 }
 
+namespace dev {
+// Zeroes a launch's work-queue counter on the launch's stream. A kernel rather than a memset, so
+// that a caller capturing the launch into a HIP graph records two kernel nodes.
+__global__ __launch_bounds__(64) void reset_counter(uint32_t* q) {
+  if (threadIdx.x == 0) q[0] = 0u;
+}
+}  // namespace dev
+
+hipError_t launch_reset_counter(uint32_t* q, hipStream_t st) {
+  hipLaunchKernelGGL(dev::reset_counter, dim3(1), dim3(64), 0, st, q);
+  return hipGetLastError();
+}
+
 hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st) {
 #define HPO_CASE(N) \
   if (ns == N) return launch_bdf_ns<N>(A, variant, grid, block, shmem, st);

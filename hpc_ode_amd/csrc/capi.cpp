@@ -167,7 +167,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   A.refine_sweeps = o.refine_sweeps;
   A.prof = reinterpret_cast<int64_t*>(d_prof);
 
-  HPO_HIP(hipMemsetAsync(res->d_queue, 0, sizeof(uint32_t), st));
+  HPO_HIP(hpo::launch_reset_counter(res->d_queue, st));
   if (tm) HPO_HIP(hipEventRecord(m->ev0, st));
   if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
   if (tm) HPO_HIP(hipEventRecord(m->ev1, st));

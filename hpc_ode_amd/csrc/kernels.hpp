@@ -68,6 +68,7 @@ int64_t wave_scratch_doubles_host(const DevMechLayout& L);
 // variant: 0 FP64 LU; 1 FP32 explicit inverse (Gauss-Jordan); 2 FP32 in-place LU
 const void* bdf_kernel_fn(int ns, int variant, int prof = 0);
 hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st);
+hipError_t launch_reset_counter(uint32_t* q, hipStream_t st);
 inline int bdf_variant(int lu_precision, int lu_layout) { return lu_precision == 0 ? 0 : (lu_layout == 0 ? 1 : 2); }
 hipError_t launch_eval(int ns, const EvalArgs& A, int grid, int block, size_t shmem, hipStream_t st);
 hipError_t launch_lu(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st);

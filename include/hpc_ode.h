@@ -106,7 +106,11 @@ int hpo_batch_get(const hpo_batch *b, int64_t first, int64_t count, double *y, i
 /* d_order: work-queue order of the cells (a permutation of 0..ncells-1) or NULL.
  * d_prof: [ncells][HPO_NPROF] shader-clock cycles per phase (RHS, Jacobian, LU factorization,
  * LU solves, whole cell) or NULL. With timing != NULL the call waits for the kernel;
- * otherwise it returns after the launch. */
+ * otherwise it returns after the launch. With timing == NULL the call enqueues two kernels on
+ * hip_stream (a work-queue counter reset and the integration) and can be captured into a HIP
+ * graph once an uncaptured call of at least as many cells has run on the same mechanism. The
+ * graph uses the mechanism's counter and scratch: its replays must not overlap other calls on
+ * that mechanism, and a later call with more cells (which grows the scratch) invalidates it. */
 #define HPO_NPROF 5
 int hpo_integrate_device(const hpo_mech *m, int64_t ncells, const double *d_y_in, double *d_y_out,
                          int64_t ld, const double *d_P, const double *d_dt, int32_t *d_status,
