@@ -1,6 +1,6 @@
-"""The GRI-3.0 1,048,576-cell dt = 1e-4 s substep with the FP32 Newton matrix and one FP64
-refinement sweep, the measured configuration, checked at its full size through properties
-that do not depend on it:
+"""The GRI-3.0 1,048,576-cell substeps with the FP32 Newton matrix and one FP64 refinement
+sweep (the measured configuration at dt = 1e-4 s, and the wide-spread one), checked at their
+full size through properties that do not depend on it:
 
 * every cell returns success;
 * element conservation: the reactor is closed, so each cell's elemental mass fractions
@@ -39,19 +39,24 @@ def _elements(y, E):
     return Yf @ E
 
 
-def test_full_size_substep_properties(gri_path):
+@pytest.mark.parametrize("workload", ["plain", "uneven"])
+def test_full_size_substep_properties(gri_path, workload):
+    """plain: T0 U[1200,2000] K, dt 1e-4 s. uneven: T0 U[300,2500] K, phi log-U[0.3,3], dt
+    log-U[1e-6,1e-3] s, per-cell step counts spread over more than 10x."""
     m = H.Mechanism(gri_path)
-    y0, P, dt = WL.gri_plain(m.species, m.W, N, seed=3)
-    print(f"integrating {N} GRI cells, FP32 Newton matrix + 1 sweep", flush=True)
+    gen = WL.gri_plain if workload == "plain" else WL.gri_uneven
+    y0, P, dt = gen(m.species, m.W, N, seed=3)
+    print(f"integrating {N} GRI cells ({workload}), FP32 Newton matrix + 1 sweep", flush=True)
     y, st, stats = m.integrate(y0, P, dt, lu_precision=1, refine_sweeps=1)
     assert (st == 0).all(), np.unique(st, return_counts=True)
     assert np.isfinite(y).all()
-    assert (y[:, 0] > 1000.0).all() and (y[:, 0] < 4000.0).all()
+    assert (y[:, 0] > 250.0).all() and (y[:, 0] < 4000.0).all()
 
     els, E = _element_matrix(gri_path, m.species, m.W)
     drift = np.abs(_elements(y, E) - _elements(y0, E)).max(axis=0)
     print("element drift max:", dict(zip(els, drift.tolist())), flush=True)
-    assert drift.max() < 1e-12      # measured on MI355X: 2.2e-15 (H), the oracle's 3.8e-16
+    # measured on MI355X: 2.2e-15 (plain) and 3.1e-13 (uneven, up to ~900 steps), both in H
+    assert drift.max() < 1e-11
 
     k = 4096
     ya, sa, _ = m.integrate(y0[:k], P[:k], dt[:k], lu_precision=1, refine_sweeps=1)
