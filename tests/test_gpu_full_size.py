@@ -39,15 +39,16 @@ def _elements(y, E):
     return Yf @ E
 
 
-@pytest.mark.parametrize("workload", ["plain", "uneven"])
-def test_full_size_substep_properties(gri_path, workload):
+@pytest.mark.parametrize("workload,n,lu_precision", [("plain", N, 1), ("uneven", N, 1), ("plain", 32768, 0)])
+def test_full_size_substep_properties(gri_path, workload, n, lu_precision):
     """plain: T0 U[1200,2000] K, dt 1e-4 s. uneven: T0 U[300,2500] K, phi log-U[0.3,3], dt
-    log-U[1e-6,1e-3] s, per-cell step counts spread over more than 10x."""
+    log-U[1e-6,1e-3] s, per-cell step counts spread over more than 10x. The 32,768-cell case is
+    the plain GPU method: FP64 everywhere, the 54x54 LU included."""
     m = H.Mechanism(gri_path)
     gen = WL.gri_plain if workload == "plain" else WL.gri_uneven
-    y0, P, dt = gen(m.species, m.W, N, seed=3)
-    print(f"integrating {N} GRI cells ({workload}), FP32 Newton matrix + 1 sweep", flush=True)
-    y, st, stats = m.integrate(y0, P, dt, lu_precision=1, refine_sweeps=1)
+    y0, P, dt = gen(m.species, m.W, n, seed=3)
+    print(f"integrating {n} GRI cells ({workload}), lu_precision {lu_precision}", flush=True)
+    y, st, stats = m.integrate(y0, P, dt, lu_precision=lu_precision, refine_sweeps=1)
     assert (st == 0).all(), np.unique(st, return_counts=True)
     assert np.isfinite(y).all()
     assert (y[:, 0] > 250.0).all() and (y[:, 0] < 4000.0).all()
@@ -59,10 +60,10 @@ def test_full_size_substep_properties(gri_path, workload):
     assert drift.max() < 1e-11
 
     k = 4096
-    ya, sa, _ = m.integrate(y0[:k], P[:k], dt[:k], lu_precision=1, refine_sweeps=1)
+    ya, sa, _ = m.integrate(y0[:k], P[:k], dt[:k], lu_precision=lu_precision, refine_sweeps=1)
     assert np.array_equal(ya, y[:k]) and np.array_equal(sa, st[:k])
 
-    idx = np.linspace(0, N - 1, 256).astype(np.int64)
+    idx = np.linspace(0, n - 1, 256).astype(np.int64)
     yo, so, _ = O.Oracle(gri_path).integrate(P[idx], dt[idx], y0[idx], nthreads=8)
     assert (so == 0).all()
     dT = np.abs(y[idx, 0] - yo[:, 0]) / yo[:, 0]
