@@ -91,3 +91,31 @@ def test_server_option_checks_come_before_device_calls(h2_path):
         assert L.hpo_server_create(arr, 1, C.byref(so), C.byref(h)) == -1
         assert not h.value
         assert field.encode() in L.hpo_last_error()
+
+
+STRUCTS = [("hpo_opts", "Opts"), ("hpo_timing", "Timing"), ("hpo_tune_report", "TuneReport"),
+           ("hpo_server_opts", "ServerOpts"), ("hpo_request_info", "RequestInfo")]
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every struct of include/hpc_ode.h has the same size and field offsets in its ctypes
+    mirror: a C program compiled against the header prints them."""
+    import os
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "hpc_ode.h"', "int main(void) {"]
+    for cname, pyname in STRUCTS:
+        lines.append(f'  printf("{pyname} size %zu\\n", sizeof({cname}));')
+        for field, _ in getattr(H, pyname)._fields_:
+            lines.append(f'  printf("{pyname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for _, pyname in STRUCTS:
+        cls = getattr(H, pyname)
+        assert got[(pyname, "size")] == C.sizeof(cls), pyname
+        for field, _ in cls._fields_:
+            assert got[(pyname, field)] == getattr(cls, field).offset, (pyname, field)
