@@ -124,9 +124,9 @@ def _errors(y, truth):
 
 
 def _agreement(y, ref, rtol=1e-6, atol=1e-10):
-    """GPU against the CPU BDF at the same tolerances, cell by cell: the north star's match
-    criterion (relative rtol on T and on every mass fraction above 1e-10) and CVODE's own
-    per-component weight (|dy| <= rtol |y| + atol)."""
+    """GPU against the CPU BDF at the same tolerances, cell by cell: the strict match criterion
+    (relative rtol on T and on every mass fraction above 1e-10) and CVODE's own per-component
+    weight (|dy| <= rtol |y| + atol)."""
     dT = np.abs(y[:, 0] - ref[:, 0]) / ref[:, 0]
     Yr, dY = ref[:, 1:], np.abs(y[:, 1:] - ref[:, 1:])
     strict = (dT <= rtol) & np.all((np.abs(Yr) <= 1e-10) | (dY <= rtol * np.abs(Yr)), axis=1)
