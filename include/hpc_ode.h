@@ -159,7 +159,8 @@ int hpo_batch_autotune(hpo_batch *b, int64_t count, int64_t sample_cells, hpo_op
  * results do not depend on what it was batched with. */
 typedef struct hpo_server hpo_server;
 typedef struct {
-  int32_t n_slots;           /* launches that may be in flight at once (one HIP stream each); 0 = 4 */
+  int32_t n_slots;           /* launches that may be in flight at once (one HIP stream each);
+                                1..8, 0 = 4 */
   int32_t cu_share_pct;      /* largest share of the device's workgroup slots one launch may
                                 take, in percent (1..100); 0 = 100. Below 100, a launch that
                                 arrives while a large one runs finds free slots at once. */
