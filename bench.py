@@ -138,28 +138,104 @@ def _agreement(y, ref, rtol=1e-6, atol=1e-10):
             "max_dy_over_rtol_y_plus_atol": float(w.max())}
 
 
-def accuracy_statement(orc, y0, P, dt, y32, y64, ncells):
-    """Errors of the GPU results on an evenly spaced subset of the workload against the same
-    BDF run on the CPU at rtol 1e-12 / atol 1e-16, beside the CPU run at the production
-    tolerance on the same cells."""
+def accuracy_statement(orc, y0, P, dt, y32, st32, nst32, y64, st64, nst64, ncells, nthreads):
+    """On an evenly spaced subset of the workload:
+
+    * errors of the GPU results and of the CPU BDF at the production tolerance against the same
+      BDF at rtol 1e-12 / atol 1e-16;
+    * the GPU against the CPU BDF at the production tolerance, cell by cell, beside the
+      round-off floor of that comparison: the CPU BDF against itself when T0, P or Y_O2 moves
+      by one ulp (oracle/floor.py). A result that is not bit-identical to the CPU's cannot
+      match on more cells than the floor does."""
+    from oracle import floor as F
     idx = np.linspace(0, len(y0) - 1, ncells).astype(np.int64)
-    nth = min(os.cpu_count() or 1, 16)
     t0 = time.perf_counter()
     truth, st_t, _ = orc.integrate(P[idx], dt[idx], y0[idx], rtol=1e-12, atol=1e-16, maxsteps=10 ** 6,
-                                   nthreads=nth)
-    y6, st6, _ = orc.integrate(P[idx], dt[idx], y0[idx], nthreads=nth)
+                                   nthreads=nthreads)
+    fl = F.roundoff_floor(orc, P[idx], dt[idx], y0[idx], nthreads=nthreads, perturbations=("T+", "P+", "O2+"))
     took = time.perf_counter() - t0
+    y6, st6 = fl["y"], fl["status"]
     ok = st_t == 0
     out = {"cells": int(ncells), "truth": "CPU BDF (same algorithm) rtol 1e-12 atol 1e-16",
            "truth_failed": int((~ok).sum()), "cpu_seconds": took,
            "fp32lu_gpu": _errors(y32[idx][ok], truth[ok]),
-           "cpu_rtol1e-6": _errors(y6[ok], truth[ok])}
+           "cpu_rtol1e-6": _errors(y6[ok], truth[ok]),
+           "roundoff_floor": F.summary(fl)}
     ok6 = st6 == 0
     out["fp32lu_gpu_vs_cpu_rtol1e-6"] = _agreement(y32[idx][ok6], y6[ok6])
+    out["fp32lu_gpu_vs_cpu_rtol1e-6"].update(F.compare(y32[idx], st32[idx], {"nst": nst32[idx]}, fl))
     if y64 is not None:
         out["fp64_gpu"] = _errors(y64[idx][ok], truth[ok])
         out["fp64_gpu_vs_cpu_rtol1e-6"] = _agreement(y64[idx][ok6], y6[ok6])
+        out["fp64_gpu_vs_cpu_rtol1e-6"].update(F.compare(y64[idx], st64[idx], {"nst": nst64[idx]}, fl))
     return out
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(orc, FL, counts, y0, P, dt, seconds, H2_PATH):
+    """The oracle's OpenMP cell loop (the shape of CVodeIntegratorOMP, schedule(runtime) over
+    cells) on this GPU's share of the host cores, in the same run as the GPU:
+
+    * a uniform random sample of the measured workload for about `seconds` (cells/s is the
+      headline figure; cells are independent, so the rate extrapolates linearly);
+    * the same on one thread;
+    * the plain GPU configuration (32,768 GRI cells, dt 1e-4 s, FP64) and the plain CPU
+      configuration (100 H2/air cells, dt 1e-3 s) timed in full."""
+    import oracle as O
+    from hpc_ode_amd import workloads as WL
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(len(os.sched_getaffinity(0)), 16)
+    omp = O.omp_info()
+    n = len(y0)
+    rng = np.random.default_rng(7)
+
+    def sample_rate(nth, budget):
+        t_cpu, done, sample = 0.0, 0, []
+        chunk = 4 * nth
+        while t_cpu < budget and done < n:
+            idx = rng.choice(n, size=chunk, replace=False)
+            tt = time.perf_counter()
+            _, so, sto = orc.integrate(P[idx], dt[idx], y0[idx], nthreads=nth)
+            t_cpu += time.perf_counter() - tt
+            done += chunk
+            sample.append(sto)
+        return done, t_cpu, {k: np.concatenate([s[k] for s in sample]) for k in sample[0]}
+
+    done, t_cpu, merged = sample_rate(nthreads, seconds)
+    done1, t1, _ = sample_rate(1, max(seconds / 2, 1.0))
+    full = {}
+    yg, Pg, dtg = WL.gri_plain(orc.mech.species, np.array(orc.mech.W), 32768, seed=3)
+    tt = time.perf_counter()
+    _, sg, _ = orc.integrate(Pg, dtg, yg, nthreads=nthreads)
+    tg = time.perf_counter() - tt
+    full["gri_32768_dt1e-4"] = {"cells": 32768, "seconds": tg, "cells_per_s": 32768 / tg,
+                                "failed": int((sg != 0).sum())}
+    o2 = O.Oracle(H2_PATH)
+    yh, Ph, dth = WL.h2_air(o2.mech.species, np.array(o2.mech.W), 100, seed=0)
+    tt = time.perf_counter()
+    _, sh, _ = o2.integrate(Ph, dth, yh, nthreads=nthreads)
+    th = time.perf_counter() - tt
+    full["h2_air_100_dt1e-3"] = {"cells": 100, "seconds": th, "cells_per_s": 100 / th,
+                                 "failed": int((sh != 0).sum())}
+    cpu = {"value": done / t_cpu, "unit": "cells/s", "cores": nthreads, "kind": "port",
+           "sample": f"{done} cells drawn uniformly from the {n}-cell workload, "
+                     f"OpenMP schedule(runtime) over cells, {t_cpu:.1f} s",
+           "one_thread": {"value": done1 / t1, "cells": done1, "seconds": t1},
+           "host": {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+                    "cpu_model": _cpu_model(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+                    "threads_used": nthreads, "schedule": omp["schedule"], "chunk": omp["chunk"],
+                    "note": "threads = OMP_NUM_THREADS, this GPU's share of the host's cores"},
+           "full_configs": full}
+    return cpu, FL.mean_cell_flops(counts, merged), nthreads
 
 
 def main():
@@ -262,11 +338,14 @@ def main():
     fp64 = None
     y64 = None
     if not a.skip_fp64:
+        # the FP64 leg runs the same cells with at most 3 timed steps and 1 warmup: it is reported
+        # beside the measured line and its steps take 1.7x as long
         o64 = H.opts(lu_precision=0, waves_per_block=o32.waves_per_block, blocks_per_cu=o32.blocks_per_cu)
-        ms64, ker64, st64, stats64, _ = run(o64, a.steps, a.warmup)
+        k64, w64 = min(a.steps, 3), min(a.warmup, 1)
+        ms64, ker64, st64, stats64, _ = run(o64, k64, w64)
         y64 = d_y.cpu().numpy()
         fp64 = dict(ms_per_step=ms64, kernel_ms=ker64, cells_per_s=n / (ms64 / 1e3),
-                    failed=int((st64 != 0).sum()))
+                    failed=int((st64 != 0).sum()), steps=k64, warmup=w64)
 
     # ---- CPU baseline and algorithmic work: the oracle on a bounded sample ----
     cpu = None
@@ -275,32 +354,17 @@ def main():
     import oracle as O
     orc = O.Oracle(MECH)
     counts = FL.mechanism_counts(orc.t)
+    nthreads = min(len(os.sched_getaffinity(0)), 16)
     if not a.no_cpu:
-        ncores = min(os.cpu_count() or 1, 16)
-        rng = np.random.default_rng(7)
-        sample = []
-        t_cpu = 0.0
-        done = 0
-        chunk = 4 * ncores
-        while t_cpu < a.cpu_seconds and done < n:
-            idx = rng.choice(n, size=chunk, replace=False)
-            tt = time.perf_counter()
-            _, so, sto = orc.integrate(P[idx], dt[idx], y0[idx], nthreads=ncores)
-            t_cpu += time.perf_counter() - tt
-            done += chunk
-            sample.append(sto)
-        merged = {k: np.concatenate([s[k] for s in sample]) for k in sample[0]}
-        cpu = {"value": done / t_cpu, "unit": "cells/s", "cores": ncores, "kind": "port",
-               "sample": f"{done} cells drawn uniformly from the {n}-cell workload, "
-                         f"OpenMP schedule(runtime) over cells, {t_cpu:.1f} s"}
-        work = FL.mean_cell_flops(counts, merged)
-    else:
-        work = None
+        cpu, work, nthreads = cpu_baseline(orc, FL, counts, y0, P, dt, a.cpu_seconds,
+                                           os.path.join(ROOT, "hpc_ode_amd", "mechanisms", "h2o2.yaml"))
 
     accuracy = None
     if a.accuracy_cells > 0 and not a.no_cpu:
         ncheck = min(a.accuracy_cells, n, 1024 if a.workload == "uneven" else n)
-        accuracy = accuracy_statement(orc, y0, P, dt, y32, y64, ncheck)
+        accuracy = accuracy_statement(orc, y0, P, dt, y32, st32, stats32[:, 0], y64,
+                                      st64 if y64 is not None else None,
+                                      stats64[:, 0] if y64 is not None else None, ncheck, nthreads)
 
     serving = None
     if a.serve_seconds > 0:
@@ -312,7 +376,7 @@ def main():
     if work is not None:
         flop_launch = work * n
         achieved = flop_launch / (ker32 / 1e3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(a.workload, n, 1),
                 "flop_per_cell": work, "kernel_ms": ker32}
     out = {

@@ -32,11 +32,11 @@ HPO_DI double rl(double v, int lane) {
 }
 HPO_DI float rl(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
 HPO_DI int rli(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
-// v_writelane: lane `lane` of `old` takes the wave-uniform value v; other lanes keep theirs.
-HPO_DI int wli(int old, int v, int lane) {
-  asm("v_writelane_b32 %0, %1, %2" : "+v"(old) : "s"(v), "i"(lane));
-  return old;
-}
+// Lane `lane` of `old` takes the wave-uniform value v; other lanes keep theirs. A compare and
+// select on the caller's (opaque) lane index rather than an inline-asm v_writelane: that
+// instruction ignores EXEC, and as an asm operand the register allocator would not know it, so
+// it could share the register with a value still live in lanes outside the current EXEC mask.
+HPO_DI int wli(int old, int v, int lane, int my_lane) { return my_lane == lane ? v : old; }
 
 template <int CTRL>
 HPO_DI double dpp(double v) {
@@ -753,7 +753,7 @@ HPO_DI bool lu_factor(LuReg<NS, F>& L, int lane_in) {
         rd = mult;
       }
     }
-    perm = wli(perm, p, k);
+    perm = wli(perm, p, k, lane);
     F l = done ? F(0) : L.a[k] * mult;
     if (!done) L.a[k] = l;
 #pragma unroll
@@ -865,7 +865,7 @@ HPO_DI bool gj_invert(LuReg<NS, float>& L, int lane_in) {
         rd = mult;
       }
     }
-    perm = wli(perm, p, k);
+    perm = wli(perm, p, k, lane);
     const float l = isp ? 0.f : L.a[k] * mult;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {

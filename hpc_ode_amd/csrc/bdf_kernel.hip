@@ -733,11 +733,13 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
 }
 
 // Small mechanisms (at most 16 state variables: a 16-register Newton row, one or two reaction
-// passes) fit several waves per SIMD, which hides the latency one wave exposes. Three waves
-// (168 registers, a few hundred bytes of spill per lane) measured fastest on 262,144 H2/air
-// cells: 861 ms FP32 / 903 ms FP64, against 918 / 915 ms at two waves and 990 / 1,170 ms at four.
+// passes) fit several waves per SIMD, which hides the latency one wave exposes. On 262,144
+// H2/air cells: 918 ms FP32 / 915 ms FP64 at two waves, 861 / 903 ms at three (168 registers,
+// a few hundred bytes of spill per lane) and 990 / 1,170 ms at four. Two is the default: one
+// three-wave FP64 build returned convergence failures for every cell after an unrelated,
+// never-taken branch was added to the cell loop, and no cause has been shown for it.
 #ifndef HPO_WAVES_PER_EU_SMALL
-#define HPO_WAVES_PER_EU_SMALL 3
+#define HPO_WAVES_PER_EU_SMALL 2
 #endif
 template <int NS, bool LU32>
 constexpr int bdf_waves_per_eu() {

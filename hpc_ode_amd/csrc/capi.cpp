@@ -52,6 +52,7 @@ hpo_mech::~hpo_mech() {
     if (d_queue) (void)hipFree(d_queue);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_last) (void)hipEventDestroy(ev_last);
   }
 }
 
@@ -66,11 +67,14 @@ void hpo_mech::ensure_device() {
   d_queue = dalloc<uint32_t>(4);
   HPO_HIP(hipEventCreate(&ev0));
   HPO_HIP(hipEventCreate(&ev1));
+  HPO_HIP(hipEventCreateWithFlags(&ev_last, hipEventDisableTiming));
   scratch_stride = hpo::wave_scratch_doubles_host(blob.lay);
 }
 
 void hpo_mech::ensure_scratch(int64_t slots) {
   if (slots <= scratch_slots) return;
+  // the previous launch on this scratch must be done before it is freed
+  if (launched) HPO_HIP(hipEventSynchronize(ev_last));
   if (d_scratch) HPO_HIP(hipFree(d_scratch));
   d_scratch = nullptr;
   d_scratch = dalloc<double>(static_cast<size_t>(slots * scratch_stride));
@@ -135,13 +139,24 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   grid = std::min<int64_t>(grid, (ncells + wpb - 1) / wpb);
   grid = std::max<int64_t>(grid, 1);
   ExecRes own;
+  std::unique_lock<std::mutex> own_lock;
+  bool own_ctx = false, capturing = false;
   if (!res) {
+    // the handle's own context: one launch at a time on the device (see hpo_mech::ev_last)
+    own_lock = std::unique_lock<std::mutex>(m->launch_mu);
+    own_ctx = true;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HPO_HIP(hipStreamIsCapturing(st, &cs));
+    capturing = cs != hipStreamCaptureStatusNone;
     m->ensure_scratch(grid * wpb);
     own.d_queue = m->d_queue;
     own.d_scratch = m->d_scratch;
     own.scratch_slots = m->scratch_slots;
     own.scratch_stride = m->scratch_stride;
     res = &own;
+    // a captured launch is not ordered against eager ones here: its replays must not overlap
+    // other launches on this handle (the header says so)
+    if (!capturing && m->launched && m->last_stream != st) HPO_HIP(hipStreamWaitEvent(st, m->ev_last, 0));
   }
   if (res->scratch_slots < grid * wpb || res->scratch_stride < m->scratch_stride)
     throw std::invalid_argument("launch context scratch too small for this launch");
@@ -171,6 +186,11 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   if (tm) HPO_HIP(hipEventRecord(m->ev0, st));
   if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
   if (tm) HPO_HIP(hipEventRecord(m->ev1, st));
+  if (own_ctx && !capturing) {
+    HPO_HIP(hipEventRecord(m->ev_last, st));
+    m->last_stream = st;
+    m->launched = true;
+  }
   if (tm) {
     HPO_HIP(hipEventSynchronize(m->ev1));
     float ms = 0.f;
@@ -421,6 +441,9 @@ static int eval_common(const hpo_mech* mc, int64_t n, const double* y, const dou
   int rc = HPO_OK;
   try {
     m->ensure_device();
+    // the handle's scratch: after any integration still running on it
+    std::lock_guard<std::mutex> g(m->launch_mu);
+    if (m->launched) HPO_HIP(hipEventSynchronize(m->ev_last));
     const int wpb = 4, block = 256;
     int64_t grid = std::min<int64_t>((n + wpb - 1) / wpb, 4096);
     m->ensure_scratch(grid * wpb);

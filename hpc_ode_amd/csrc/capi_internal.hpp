@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -74,6 +75,13 @@ struct hpo_mech {
   int64_t scratch_stride = 0;
   uint32_t* d_queue = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Launches on the handle's own queue counter and scratch are ordered on the device: each
+  // waits for the previous one (ev_last, recorded on last_stream) when it is issued on another
+  // stream, and host threads take launch_mu around the issue.
+  std::mutex launch_mu;
+  hipEvent_t ev_last = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool launched = false;
 
   ~hpo_mech();
   void ensure_device();               // uploads the tables on first use (current device)

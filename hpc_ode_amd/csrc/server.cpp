@@ -1,16 +1,26 @@
 // Serving loop (include/hpc_ode.h, hpo_server_*): continuous batching of chemistry-substep
 // requests over device-resident cell streams.
 //
-// Device memory: every stream owns its rows y [n][K], pressures P [n] and statuses [n] from
-// open to close (admitted against the HBM budget). Every launch slot owns a HIP stream, a
-// work-queue counter, per-wave scratch sized for a whole-device launch of any of the server's
-// mechanisms, and packing buffers of max_batch_cells rows (device) plus pinned host staging.
+// Device memory: every stream holds its rows y [n][K], pressures P [n] and statuses [n] from
+// open to close (admitted against the HBM budget) in one block from the server's pool. The
+// pool keeps freed blocks in power-of-two size classes and hands them out again, so opening
+// and closing streams under load neither allocates nor frees device memory once the classes
+// in use are warm (a hipFree waits for the whole device, which here is kept busy by the
+// launches). Every launch slot owns a HIP stream, a work-queue counter, per-wave scratch
+// sized for a whole-device launch of any of the server's mechanisms, and packing buffers of
+// max_batch_cells rows (device) plus pinned host staging. Stream uploads and downloads run on
+// a dedicated non-blocking stream, never on the null stream.
 //
 // A launch packs its requests' rows into the slot buffers (device-to-device), integrates them
 // in place with the persistent BDF kernel, scatters them back to their streams and stages the
 // rows callers asked for in pinned memory; an event marks the end. The scheduler thread polls
 // the slots' events, completes finished batches (copies staged rows to caller buffers, wakes
-// waiters) and fills free slots from the pending queue in arrival order.
+// waiters) and fills free slots from the pending queue in arrival order. It holds the server
+// lock only to move requests between queues: the HIP calls of a launch and the host copies of
+// a completion run unlocked (their streams are in flight, so no client call touches them).
+//
+// Lifetime: a request holds a reference to its stream, so a stream closed between the
+// completion of its substep and the wait for it stays valid until that wait returns.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -32,18 +42,61 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// Device blocks in power-of-two size classes (at least 64 KiB), reused after release and
+// freed only when the server is destroyed.
+struct BlockPool {
+  std::mutex mu;
+  std::map<int64_t, std::vector<void*>> free_blocks;   // class bytes -> blocks
+  std::vector<void*> all;
+  static int64_t class_of(int64_t bytes) {
+    int64_t c = 64 << 10;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  void* get(int64_t cls) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto& v = free_blocks[cls];
+      if (!v.empty()) {
+        void* p = v.back();
+        v.pop_back();
+        return p;
+      }
+    }
+    void* p = dalloc<uint8_t>(static_cast<size_t>(cls));
+    std::lock_guard<std::mutex> g(mu);
+    all.push_back(p);
+    return p;
+  }
+  void put(void* p, int64_t cls) {
+    std::lock_guard<std::mutex> g(mu);
+    free_blocks[cls].push_back(p);
+  }
+  void release() {
+    for (void* p : all) (void)hipFree(p);
+    all.clear();
+    free_blocks.clear();
+  }
+};
+
 struct SrvStream {
   int64_t id = 0;
   int mech = 0;
-  int64_t n = 0, ld = 0, bytes = 0;
+  int64_t n = 0, ld = 0, bytes = 0;   // bytes: the pool class charged to the budget
+  void* block = nullptr;
   double *d_y = nullptr, *d_P = nullptr;
   int32_t* d_status = nullptr;
   bool in_flight = false;   // a substep of this stream is pending or running
+  BlockPool* pool = nullptr;
+  ~SrvStream() {
+    if (block && pool) pool->put(block, bytes);
+  }
 };
 
 struct SrvReq {
   int64_t id = 0;
-  SrvStream* st = nullptr;
+  std::shared_ptr<SrvStream> st;
+  int64_t ncells = 0;
   double dt = 0;
   double* y_out = nullptr;
   int64_t ld_out = 0;
@@ -76,7 +129,10 @@ struct hpo_server {
   hpo_server_opts opts{};
   int device = 0;
   std::vector<Slot> slots;
-  std::map<int64_t, std::unique_ptr<SrvStream>> streams;
+  BlockPool pool;
+  hipStream_t io = nullptr;   // uploads and downloads of stream state
+  std::mutex io_mu;
+  std::map<int64_t, std::shared_ptr<SrvStream>> streams;
   std::map<int64_t, std::unique_ptr<SrvReq>> reqs;
   std::deque<SrvReq*> pending;
   int64_t next_stream = 1, next_req = 1;
@@ -99,19 +155,18 @@ struct hpo_server {
       if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     slots.clear();
-    for (auto& kv : streams) free_stream(*kv.second);
+    reqs.clear();
+    pending.clear();
     streams.clear();
+    if (io) (void)hipStreamDestroy(io);
+    io = nullptr;
+    pool.release();
   }
 
-  static void free_stream(SrvStream& s) {
-    for (void* p : {static_cast<void*>(s.d_y), static_cast<void*>(s.d_P), static_cast<void*>(s.d_status)})
-      if (p) (void)hipFree(p);
-    s.d_y = s.d_P = nullptr;
-    s.d_status = nullptr;
-  }
-
-  // Packs a batch into slot k and launches it (called with the lock held).
-  void launch(int k, std::vector<SrvReq*>&& batch) {
+  // Packs a batch into slot k and launches it. Called by the scheduler thread WITHOUT the
+  // lock: the batch's requests are out of the pending queue and their streams are in flight
+  // (close and stream_get refuse them), so nothing here is shared with client calls.
+  void launch(int k, const std::vector<SrvReq*>& batch) {
     Slot& sl = slots[k];
     hpo_mech* m = mechs[batch.front()->st->mech];
     const int64_t ld = m->blob.lay.K;
@@ -146,11 +201,11 @@ struct hpo_server {
     }
     HPO_HIP(hipMemcpyAsync(sl.h_status, sl.d_status, total * sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
     HPO_HIP(hipEventRecord(sl.done, sl.stream));
-    sl.batch = std::move(batch);
-    sl.busy = true;
   }
 
-  // Completes the finished batch of slot k (lock held).
+  // Copies the finished batch of slot k to the callers' buffers. Called WITHOUT the lock (the
+  // requests are not yet marked done, so their callers are still waiting); finish() then
+  // publishes the results under the lock.
   void complete(int k) {
     Slot& sl = slots[k];
     const auto now = Clock::now();
@@ -165,8 +220,15 @@ struct hpo_server {
       if (r->status_out) std::memcpy(r->status_out, sl.h_status + r->offset, st.n * sizeof(int32_t));
       r->failed = failed;
       r->t_done = now;
+    }
+  }
+
+  // Marks slot k's requests done and frees the slot (lock held).
+  void finish(int k) {
+    Slot& sl = slots[k];
+    for (SrvReq* r : sl.batch) {
       r->done = true;
-      st.in_flight = false;
+      r->st->in_flight = false;
     }
     sl.batch.clear();
     sl.busy = false;
@@ -201,9 +263,12 @@ struct hpo_server {
         bool any_busy = false;
         for (int k = 0; k < static_cast<int>(slots.size()); ++k) {
           if (!slots[k].busy) continue;
+          lk.unlock();
           hipError_t q = hipEventQuery(slots[k].done);
+          if (q == hipSuccess) complete(k);
+          lk.lock();
           if (q == hipSuccess)
-            complete(k);
+            finish(k);
           else if (q == hipErrorNotReady)
             any_busy = true;
           else
@@ -216,7 +281,11 @@ struct hpo_server {
             if (slots[k].busy) continue;
             std::vector<SrvReq*> b = take_batch(slots[k].cap);
             if (b.empty()) break;
-            launch(k, std::move(b));
+            slots[k].batch = std::move(b);
+            slots[k].busy = true;
+            lk.unlock();
+            launch(k, slots[k].batch);
+            lk.lock();
             launched = true;
             any_busy = true;
           }
@@ -231,6 +300,7 @@ struct hpo_server {
         }
       }
     } catch (const std::exception& e) {
+      if (!lk.owns_lock()) lk.lock();
       thread_error = e.what();
       stop = true;
       for (auto& kv : reqs) kv.second->done = true;
@@ -257,9 +327,7 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
     hpo_server_opts_default(&s->opts);
     if (o) s->opts = *o;
     if (s->opts.n_slots <= 0) s->opts.n_slots = 4;
-    // 8 is the most slots measured under load; a 16-slot run at 70 % load stopped reporting
-    // for minutes (cause not found), so larger counts are refused.
-    if (s->opts.n_slots > 8) throw std::invalid_argument("n_slots must be at most 8");
+    if (s->opts.n_slots > 16) throw std::invalid_argument("n_slots must be at most 16");
     if (s->opts.max_batch_cells <= 0) s->opts.max_batch_cells = 262144;
     if (s->opts.cu_share_pct == 0) s->opts.cu_share_pct = 100;
     if (s->opts.cu_share_pct < 1 || s->opts.cu_share_pct > 100)
@@ -280,6 +348,7 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
     }
     size_t fr = 0, to = 0;
     HPO_HIP(hipMemGetInfo(&fr, &to));
+    HPO_HIP(hipStreamCreateWithFlags(&s->io, hipStreamNonBlocking));
     s->slots.resize(s->opts.n_slots);
     for (Slot& sl : s->slots) {
       sl.cap = s->opts.max_batch_cells;
@@ -329,13 +398,16 @@ int hpo_server_open_stream(hpo_server* s, int32_t mech, int64_t n, const double*
   const int64_t K = s->mechs[mech]->blob.lay.K;
   if (ld < K) return fail(HPO_E_INVALID, "ld smaller than the state length");
   if (n > s->opts.max_batch_cells) return fail(HPO_E_INVALID, "stream larger than a launch (max_batch_cells)");
-  const int64_t bytes = n * (K * 8 + 8 + 4);
-  auto st = std::make_unique<SrvStream>();
+  // y [n][K], then P [n], then status [n], each 256-byte aligned inside one pool block
+  const int64_t y_bytes = (n * K * 8 + 255) & ~int64_t(255);
+  const int64_t p_bytes = (n * 8 + 255) & ~int64_t(255);
+  const int64_t cls = BlockPool::class_of(y_bytes + p_bytes + n * 4);
+  auto st = std::make_shared<SrvStream>();
   {
     std::lock_guard<std::mutex> g(s->mu);
     if (!s->thread_error.empty()) return fail(HPO_E_DEVICE, s->thread_error);
-    if (s->used + bytes > s->budget) return fail(HPO_E_BUSY, "HBM budget of the server exhausted");
-    s->used += bytes;
+    if (s->used + cls > s->budget) return fail(HPO_E_BUSY, "HBM budget of the server exhausted");
+    s->used += cls;
     st->id = s->next_stream++;
   }
   try {
@@ -343,20 +415,26 @@ int hpo_server_open_stream(hpo_server* s, int32_t mech, int64_t n, const double*
     st->mech = mech;
     st->n = n;
     st->ld = K;
-    st->bytes = bytes;
-    st->d_y = dalloc<double>(static_cast<size_t>(n * K));
-    st->d_P = dalloc<double>(n);
-    st->d_status = dalloc<int32_t>(n);
-    HPO_HIP(hipMemcpy2D(st->d_y, K * sizeof(double), y0, ld * sizeof(double), K * sizeof(double), n,
-                        hipMemcpyHostToDevice));
+    st->bytes = cls;
+    st->block = s->pool.get(cls);
+    st->pool = &s->pool;
+    uint8_t* base = static_cast<uint8_t*>(st->block);
+    st->d_y = reinterpret_cast<double*>(base);
+    st->d_P = reinterpret_cast<double*>(base + y_bytes);
+    st->d_status = reinterpret_cast<int32_t*>(base + y_bytes + p_bytes);
     std::vector<double> Pv;
     if (!P) Pv.assign(n, P_scalar);
-    HPO_HIP(hipMemcpy(st->d_P, P ? P : Pv.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    std::lock_guard<std::mutex> g(s->io_mu);
+    HPO_HIP(hipMemcpy2DAsync(st->d_y, K * sizeof(double), y0, ld * sizeof(double), K * sizeof(double), n,
+                             hipMemcpyHostToDevice, s->io));
+    HPO_HIP(hipMemcpyAsync(st->d_P, P ? P : Pv.data(), n * sizeof(double), hipMemcpyHostToDevice, s->io));
+    HPO_HIP(hipStreamSynchronize(s->io));
   } catch (...) {
-    hpo_server::free_stream(*st);
+    int rc = map_exceptions();
+    st.reset();   // returns the block to the pool
     std::lock_guard<std::mutex> g(s->mu);
-    s->used -= bytes;
-    return map_exceptions();
+    s->used -= cls;
+    return rc;
   }
   std::lock_guard<std::mutex> g(s->mu);
   *id = st->id;
@@ -366,7 +444,7 @@ int hpo_server_open_stream(hpo_server* s, int32_t mech, int64_t n, const double*
 
 int hpo_server_close_stream(hpo_server* s, int64_t id) {
   if (!s) return fail(HPO_E_INVALID, "null server");
-  std::unique_ptr<SrvStream> st;
+  std::shared_ptr<SrvStream> st;
   {
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->streams.find(id);
@@ -376,8 +454,8 @@ int hpo_server_close_stream(hpo_server* s, int64_t id) {
     s->streams.erase(it);
     s->used -= st->bytes;
   }
-  (void)hipSetDevice(s->device);
-  hpo_server::free_stream(*st);
+  // the block returns to the pool when the last reference (this one, or a request not yet
+  // waited for) goes
   return HPO_OK;
 }
 
@@ -394,7 +472,8 @@ int hpo_server_submit(hpo_server* s, int64_t sid, double dt, double* y_out, int6
   if (st.in_flight) return fail(HPO_E_BUSY, "the previous substep of this stream has not completed");
   auto r = std::make_unique<SrvReq>();
   r->id = s->next_req++;
-  r->st = &st;
+  r->st = it->second;
+  r->ncells = st.n;
   r->dt = dt;
   r->y_out = y_out;
   r->ld_out = ld_out;
@@ -424,7 +503,7 @@ int hpo_server_wait(hpo_server* s, int64_t rid, double timeout_s, hpo_request_in
     info->latency_ms = ms_between(r->t_submit, r->t_done);
     info->queue_ms = ms_between(r->t_submit, r->t_launch);
     info->run_ms = ms_between(r->t_launch, r->t_done);
-    info->ncells = r->st->n;
+    info->ncells = r->ncells;
     info->batch_cells = r->batch_cells;
     info->batch_requests = r->batch_requests;
     info->failed_cells = r->failed;
@@ -435,23 +514,33 @@ int hpo_server_wait(hpo_server* s, int64_t rid, double timeout_s, hpo_request_in
 
 int hpo_server_stream_get(hpo_server* s, int64_t sid, double* y, int64_t ld) {
   if (!s || !y) return fail(HPO_E_INVALID, "bad argument");
-  SrvStream* st = nullptr;
+  std::shared_ptr<SrvStream> st;
   {
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->streams.find(sid);
     if (it == s->streams.end()) return fail(HPO_E_INVALID, "no such stream");
-    st = it->second.get();
+    st = it->second;
     if (st->in_flight) return fail(HPO_E_BUSY, "a substep of this stream is in flight");
+    // held in flight while the copy runs, so no substep is submitted over it
+    st->in_flight = true;
   }
-  if (ld < st->ld) return fail(HPO_E_INVALID, "ld smaller than the state length");
-  try {
-    HPO_HIP(hipSetDevice(s->device));
-    HPO_HIP(hipMemcpy2D(y, ld * sizeof(double), st->d_y, st->ld * sizeof(double), st->ld * sizeof(double), st->n,
-                        hipMemcpyDeviceToHost));
-    return HPO_OK;
-  } catch (...) {
-    return map_exceptions();
+  int rc = HPO_OK;
+  if (ld < st->ld) {
+    rc = fail(HPO_E_INVALID, "ld smaller than the state length");
+  } else {
+    try {
+      HPO_HIP(hipSetDevice(s->device));
+      std::lock_guard<std::mutex> g(s->io_mu);
+      HPO_HIP(hipMemcpy2DAsync(y, ld * sizeof(double), st->d_y, st->ld * sizeof(double), st->ld * sizeof(double),
+                               st->n, hipMemcpyDeviceToHost, s->io));
+      HPO_HIP(hipStreamSynchronize(s->io));
+    } catch (...) {
+      rc = map_exceptions();
+    }
   }
+  std::lock_guard<std::mutex> g(s->mu);
+  st->in_flight = false;
+  return rc;
 }
 
 int hpo_server_hbm_used(const hpo_server* s, int64_t* used, int64_t* budget) {

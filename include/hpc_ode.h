@@ -110,7 +110,11 @@ int hpo_batch_get(const hpo_batch *b, int64_t first, int64_t count, double *y, i
  * hip_stream (a work-queue counter reset and the integration) and can be captured into a HIP
  * graph once an uncaptured call of at least as many cells has run on the same mechanism. The
  * graph uses the mechanism's counter and scratch: its replays must not overlap other calls on
- * that mechanism, and a later call with more cells (which grows the scratch) invalidates it. */
+ * that mechanism, and a later call with more cells (which grows the scratch) invalidates it.
+ * Uncaptured calls on one mechanism may come from several host threads and streams: they share
+ * the mechanism's counter and scratch, so each launch waits on the device for the previous one
+ * (an event, when the stream differs) and they never overlap. Launches that must overlap need
+ * separate mechanism handles (or the server, whose slots own their counters and scratch). */
 #define HPO_NPROF 5
 int hpo_integrate_device(const hpo_mech *m, int64_t ncells, const double *d_y_in, double *d_y_out,
                          int64_t ld, const double *d_P, const double *d_dt, int32_t *d_status,
@@ -160,7 +164,7 @@ int hpo_batch_autotune(hpo_batch *b, int64_t count, int64_t sample_cells, hpo_op
 typedef struct hpo_server hpo_server;
 typedef struct {
   int32_t n_slots;           /* launches that may be in flight at once (one HIP stream each);
-                                1..8, 0 = 4 */
+                                1..16, 0 = 4 */
   int32_t cu_share_pct;      /* largest share of the device's workgroup slots one launch may
                                 take, in percent (1..100); 0 = 100. Below 100, a launch that
                                 arrives while a large one runs finds free slots at once. */

@@ -78,8 +78,17 @@ def lib():
                                     C.POINTER(OrOpts), C.POINTER(OrStats)]
         L.or_cvode_batch.argtypes = [C.POINTER(OrMech), C.c_long, _dp, _dp, _dp,
                                      C.POINTER(OrOpts), _ip, C.POINTER(OrStats), C.c_int]
+        L.or_omp_info.argtypes = [_ip, _ip, _ip]
         _lib = L
     return _lib
+
+
+def omp_info() -> dict:
+    """Default OpenMP thread count and the schedule(runtime) kind and chunk of the cell loop."""
+    t, k, c = C.c_int(), C.c_int(), C.c_int()
+    lib().or_omp_info(C.byref(t), C.byref(k), C.byref(c))
+    kinds = {1: "static", 2: "dynamic", 3: "guided", 4: "auto"}
+    return {"max_threads": t.value, "schedule": kinds.get(k.value & 0x7FFFFFFF, str(k.value)), "chunk": c.value}
 
 
 def _p(a, t=_dp):

@@ -1190,6 +1190,21 @@ int or_cvode_cell(const or_mech *m, double P, double dt, double *y, const or_opt
   return or_cvode_generic(m->K, chem_f, chem_j, &u, 0.0, dt, y, o, st);
 }
 
+void or_omp_info(int *max_threads, int *sched_kind, int *chunk) {
+#ifdef _OPENMP
+  omp_sched_t k;
+  int c;
+  omp_get_schedule(&k, &c);
+  *max_threads = omp_get_max_threads();
+  *sched_kind = (int)k;
+  *chunk = c;
+#else
+  *max_threads = 1;
+  *sched_kind = 0;
+  *chunk = 0;
+#endif
+}
+
 void or_cvode_batch(const or_mech *m, long ncells, const double *P, const double *dt,
                     double *y, const or_opts *o, int *status, or_stats *stats, int nthreads) {
 #ifdef _OPENMP

@@ -96,6 +96,10 @@ int or_cvode_cell(const or_mech *m, double P, double dt, double *y, const or_opt
 void or_cvode_batch(const or_mech *m, long ncells, const double *P, const double *dt,
                     double *y, const or_opts *o, int *status, or_stats *stats, int nthreads);
 
+/* The OpenMP runtime the cell loop runs under: the default thread count and the
+ * schedule(runtime) setting (omp_sched_t kind: 1 static, 2 dynamic, 3 guided, 4 auto; chunk). */
+void or_omp_info(int *max_threads, int *sched_kind, int *chunk);
+
 #ifdef __cplusplus
 }
 #endif

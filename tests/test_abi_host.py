@@ -82,7 +82,7 @@ def test_server_option_checks_come_before_device_calls(h2_path):
     m = H.Mechanism(h2_path)
     L = H.lib()
     arr = (C.c_void_p * 1)(m.handle.value)
-    for field, bad in (("cu_share_pct", 101), ("cu_share_pct", -5), ("n_slots", 9)):
+    for field, bad in (("cu_share_pct", 101), ("cu_share_pct", -5), ("n_slots", 17)):
         so = H.ServerOpts()
         L.hpo_server_opts_default(C.byref(so))
         assert so.cu_share_pct == 0 and so.n_slots == 4

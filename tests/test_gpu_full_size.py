@@ -9,13 +9,16 @@ full size through properties that do not depend on it:
   rounding only (the oracle's is below 4e-16);
 * the cells integrated alone give bit-identical rows to the same cells inside the full batch
   (one wave per cell runs the same instructions whatever else is in the launch);
-* 256 evenly spaced cells agree with the oracle at the bar of the small parity tests."""
+* 256 evenly spaced cells agree with the oracle at the stated bar (T and every mass fraction
+  above 1e-10 within a relative 1e-6) as often as the oracle agrees with itself under one-ulp
+  input changes (oracle/floor.py)."""
 import numpy as np
 import pytest
 import yaml
 
 import hpc_ode_amd as H
 import oracle as O
+from oracle import floor as F
 from hpc_ode_amd import workloads as WL
 from oracle import mech as OM
 
@@ -43,7 +46,8 @@ def _elements(y, E):
 def test_full_size_substep_properties(gri_path, workload, n, lu_precision):
     """plain: T0 U[1200,2000] K, dt 1e-4 s. uneven: T0 U[300,2500] K, phi log-U[0.3,3], dt
     log-U[1e-6,1e-3] s, per-cell step counts spread over more than 10x. The 32,768-cell case is
-    the plain GPU method: FP64 everywhere, the 54x54 LU included."""
+    the plain GPU method: FP64 everywhere, the 53x53 Newton-matrix LU included (state
+    [T, Y_0..Y_51])."""
     m = H.Mechanism(gri_path)
     gen = WL.gri_plain if workload == "plain" else WL.gri_uneven
     y0, P, dt = gen(m.species, m.W, n, seed=3)
@@ -64,10 +68,12 @@ def test_full_size_substep_properties(gri_path, workload, n, lu_precision):
     assert np.array_equal(ya, y[:k]) and np.array_equal(sa, st[:k])
 
     idx = np.linspace(0, n - 1, 256).astype(np.int64)
-    yo, so, _ = O.Oracle(gri_path).integrate(P[idx], dt[idx], y0[idx], nthreads=8)
-    assert (so == 0).all()
+    fl = F.roundoff_floor(O.Oracle(gri_path), P[idx], dt[idx], y0[idx], nthreads=8)
+    assert (fl["status"] == 0).all()
+    yo = fl["y"]
     dT = np.abs(y[idx, 0] - yo[:, 0]) / yo[:, 0]
     w = np.abs(y[idx] - yo) / (1e-6 * np.abs(yo) + 1e-10)
     print(f"oracle sample: T max rel {dT.max():.2e}, median {np.median(dT):.2e}, max weighted {w.max():.1f}")
+    print("floor", F.summary(fl), "gpu", F.compare(y[idx], st[idx], {k: v[idx] for k, v in stats.items()}, fl))
+    F.check_against_floor(y[idx], st[idx], {k: v[idx] for k, v in stats.items()}, fl)
     assert dT.max() < 1e-5 and w.max() < 300.0
-    assert np.median(dT) < 1e-8

@@ -1,8 +1,12 @@
 """GPU parity: HIP kernels (through the C ABI) against the CPU oracle on the same inputs.
 
-Tolerances are written per test: component evaluations (RHS, Jacobian, LU) must agree to
-floating-point noise; integrated states to the bar the integration tolerance allows
-(rtol 1e-6 on T and on mass fractions above 1e-10).
+Tolerances are written per test. Component evaluations (RHS, Jacobian, LU) agree to
+floating-point noise. Integrated states are judged at the stated bar, T and every mass
+fraction above 1e-10 within a relative 1e-6 of the oracle, against the round-off floor of
+that bar measured on the same cells (oracle/floor.py: the oracle against itself under
+one-ulp input changes). The GPU must match on at least the floor's share of cells, on the
+cells the floor never breaks, and keep the oracle's step count as often as the floor does.
+A weighted bound, |dy| <= 300 (rtol |y| + atol), stays as a sanity check on every cell.
 """
 import numpy as np
 import pytest
@@ -164,13 +168,9 @@ def test_lu_block_layout_rejects_fp32():
         H.lu_solve(M, b, fp32=True, layout=1)
 
 
-def _compare_states(yg, yo, T_rtol, w_max, rtol=1e-6, atol=1e-10):
-    """GPU vs oracle on the same inputs. T: relative difference below T_rtol. All state
-    components: |dy| / (rtol |y| + atol) below w_max. Both integrations run the same BDF
-    algorithm; on igniting cells the FP-level differences (exp/log implementations,
-    summation order) are amplified along the trajectory, so the bar is a multiple of the
-    integration tolerance, not a few ulps. Their accuracy against an rtol-1e-12 solution is
-    checked separately (test_accuracy_vs_tight_reference)."""
+def _sanity(yg, yo, T_rtol=1e-5, w_max=300.0, rtol=1e-6, atol=1e-10):
+    """Every cell, whatever the floor: T within T_rtol and every component within
+    w_max (rtol |y| + atol) of the oracle."""
     dT = np.abs(yg[:, 0] - yo[:, 0]) / yo[:, 0]
     assert dT.max() < T_rtol, f"T max rel diff {dT.max():.3e}"
     w = np.abs(yg - yo) / (rtol * np.abs(yo) + atol)
@@ -178,18 +178,31 @@ def _compare_states(yg, yo, T_rtol, w_max, rtol=1e-6, atol=1e-10):
     return dT, w
 
 
-def test_integrate_gri_matches_oracle(gri):
+def _floor_check(o, m, y0, P, dt, **kw):
+    """GPU result against the oracle at the stated bar, judged against the round-off floor
+    measured on the same cells; prints the figures so a failure names them."""
+    from oracle import floor as F
+    fl = F.roundoff_floor(o, P, dt, y0, nthreads=8)
+    assert (fl["status"] == 0).all()
+    yg, sg, stg = m.integrate(y0, P, dt, **kw)
+    assert (sg == 0).all()
+    print("floor", F.summary(fl), "gpu", F.compare(yg, sg, stg, fl), flush=True)
+    c = F.check_against_floor(yg, sg, stg, fl)
+    _sanity(yg, fl["y"])
+    return yg, stg, fl, c
+
+
+@pytest.mark.parametrize("lu_precision", [0, 1])
+def test_integrate_gri_matches_oracle(gri, lu_precision):
+    """256 plain GRI cells (T0 U[1200,2000] K, dt 1e-4 s): FP64 LU, and the FP32 inverse with
+    one FP64 refinement sweep, at the stated bar against the round-off floor."""
     m, o = gri
-    y0, P, dt = WL.gri_plain(m.species, m.W, 96, seed=11)
-    yg, sg, stg = m.integrate(y0, P, dt)
-    yo, so, sto = o.integrate(P, dt, y0, nthreads=8)
-    assert (sg == 0).all() and (so == 0).all()
-    dT, w = _compare_states(yg, yo, 1e-5, 300.0)
-    assert np.median(dT) < 1e-8
+    y0, P, dt = WL.gri_plain(m.species, m.W, 256, seed=11)
+    yg, stg, fl, c = _floor_check(o, m, y0, P, dt, lu_precision=lu_precision, refine_sweeps=1)
+    sto = fl["stats"]
     # the same algorithm: per-cell work counters agree on average
     for f in ("nst", "nfe", "nje"):
         assert abs(stg[f].mean() - sto[f].mean()) <= 0.02 * sto[f].mean() + 0.5, f
-    assert (stg["nst"] == sto["nst"]).mean() > 0.3
 
 
 @pytest.mark.parametrize("lu_precision,sweeps", [(0, 1), (1, 1), (1, 2)])
@@ -198,10 +211,7 @@ def test_integrate_h2_matches_oracle(h2, lu_precision, sweeps):
     (the refinement's J x reads its 10 columns as one group, no padding)."""
     m, o = h2
     y0, P, dt = WL.h2_air(m.species, m.W, 100, seed=12)
-    yg, sg, stg = m.integrate(y0, P, dt, lu_precision=lu_precision, refine_sweeps=sweeps)
-    yo, so, sto = o.integrate(P, dt, y0, nthreads=8)
-    assert (sg == 0).all() and (so == 0).all()
-    _compare_states(yg, yo, 1e-5, 100.0)
+    _floor_check(o, m, y0, P, dt, lu_precision=lu_precision, refine_sweeps=sweeps)
 
 
 @pytest.mark.parametrize("lu_precision", [0, 1])
@@ -209,22 +219,28 @@ def test_integrate_h2li_xml_mechanism(h2li_path, lu_precision):
     """The 9-variable kernel (h2_li_2004.xml) integrated against the oracle in both precisions."""
     import oracle as O
     m, o = H.Mechanism(h2li_path), O.Oracle(h2li_path)
-    y0, P, dt = WL.h2_air(m.species, m.W, 64, seed=43)
-    yg, sg, _ = m.integrate(y0, P, dt, lu_precision=lu_precision)
-    yo, so, _ = o.integrate(P, dt, y0, nthreads=8)
-    assert (sg == 0).all() and (so == 0).all()
-    _compare_states(yg, yo, 1e-5, 100.0)
+    y0, P, dt = WL.h2_air(m.species, m.W, 128, seed=43)
+    _floor_check(o, m, y0, P, dt, lu_precision=lu_precision, refine_sweeps=1)
 
 
-@pytest.mark.parametrize("sweeps,layout", [(0, 0), (1, 0), (2, 0), (1, 1)])
+@pytest.mark.parametrize("sweeps,layout", [(1, 0), (2, 0), (1, 1)])
 def test_integrate_gri_fp32_lu(gri, sweeps, layout):
     """FP32 Newton matrix (layout 0: explicit inverse, 1: in-place LU) with FP64 refinement."""
     m, o = gri
+    y0, P, dt = WL.gri_plain(m.species, m.W, 256, seed=13)
+    _floor_check(o, m, y0, P, dt, lu_precision=1, refine_sweeps=sweeps, lu_layout=layout)
+
+
+def test_integrate_gri_fp32_lu_without_refinement(gri):
+    """The FP32 inverse with no refinement sweep is an option, not the configured method: on the
+    CPU the same choice falls below the floor (tests/test_parity_floor.py), so here it is held
+    only to the sanity bound and to the oracle's mean step count."""
+    m, o = gri
     y0, P, dt = WL.gri_plain(m.species, m.W, 64, seed=13)
-    yg, sg, stg = m.integrate(y0, P, dt, lu_precision=1, refine_sweeps=sweeps, lu_layout=layout)
+    yg, sg, stg = m.integrate(y0, P, dt, lu_precision=1, refine_sweeps=0)
     yo, so, sto = o.integrate(P, dt, y0, nthreads=8)
     assert (sg == 0).all()
-    _compare_states(yg, yo, 1e-5, 300.0)
+    _sanity(yg, yo)
     assert abs(stg["nst"].mean() - sto["nst"].mean()) <= 0.03 * sto["nst"].mean()
 
 

@@ -74,20 +74,52 @@ def test_substeps_continue_from_resident_state(mechs):
 
 
 def test_admission_against_hbm_budget(mechs):
+    """A stream is charged the pool block that holds it: its rows, pressures and statuses
+    (53 x 8 + 8 + 4 bytes per GRI cell, 256-byte aligned parts) rounded up to a power of two,
+    here 512 KiB for 600 cells. A budget of 1,000,000 bytes admits one such stream, not two."""
     gri = mechs[1]
-    per_cell = gri.n_state * 8 + 12
-    srv = H.Server(mechs, hbm_budget_bytes=1000 * per_cell)
+    srv = H.Server(mechs, hbm_budget_bytes=1_000_000)
     try:
         y, P = _state(gri, 1, 600, 1)
         a = srv.open_stream(1, y, P)
         assert a is not None
-        assert srv.open_stream(1, y, P) is None          # 1200 cells > budget of 1000
+        assert srv.open_stream(1, y, P) is None
         used, budget = srv.hbm()
-        assert used == 600 * per_cell and budget == 1000 * per_cell
+        assert used == 1 << 19 and budget == 1_000_000
         srv.close_stream(a)
-        b = srv.open_stream(1, y, P)
+        assert srv.hbm()[0] == 0
+        b = srv.open_stream(1, y, P)          # reuses the pooled block
         assert b is not None
+        assert np.array_equal(srv.stream_state(b, 600, gri.n_state), y)
         srv.close_stream(b)
+    finally:
+        srv.close()
+
+
+def test_wait_after_close_of_a_completed_stream(mechs):
+    """A stream may be closed once its substep has completed and before that request is waited
+    for; the wait still reports the request (the request keeps the stream alive)."""
+    srv = H.Server(mechs)
+    try:
+        h2 = mechs[0]
+        y, P = _state(h2, 0, 300, 4)
+        sid = srv.open_stream(0, y, P)
+        rid = srv.submit(sid, 2e-5, want_state=False)
+        import time
+        for _ in range(20000):
+            try:
+                srv.close_stream(sid)
+                break
+            except H.HpoError:
+                time.sleep(0.001)
+        else:
+            raise AssertionError("the substep did not complete")
+        _, _, info = srv.wait(rid)
+        assert info["ncells"] == 300 and info["failed_cells"] == 0
+        # the freed block serves the next stream
+        sid2 = srv.open_stream(0, y, P)
+        assert np.array_equal(srv.stream_state(sid2, 300, h2.n_state), y)
+        srv.close_stream(sid2)
     finally:
         srv.close()
 
@@ -134,5 +166,18 @@ def test_short_poisson_load(mechs):
         assert r["substeps_completed"] == 2 * r["requests"]
         assert r["failed_cells"] == 0
         assert r["latency_ms_p99"] >= r["latency_ms_p50"] > 0
+    finally:
+        srv.close()
+
+
+def test_sixteen_slots_under_load(mechs):
+    """16 launch slots (16 HIP streams over the process's hardware queues) under a Poisson load
+    with opens and closes from many client threads: every substep completes."""
+    srv = H.Server(mechs, n_slots=16)
+    try:
+        r = SV.run_load(srv, mechs, rate_hz=40.0, duration_s=3.0, seed=5, substeps=2, max_cells=16384)
+        print(r, flush=True)
+        assert r["substeps_completed"] == 2 * r["requests"]
+        assert r["failed_cells"] == 0
     finally:
         srv.close()

@@ -18,11 +18,18 @@ def gri(gri_path):
 
 @pytest.mark.parametrize("lu_precision", [0, 1])
 def test_uneven_cells_match_oracle(gri, lu_precision):
+    """256 wide-spread cells at the stated bar (T and every mass fraction above 1e-10 within a
+    relative 1e-6 of the oracle), judged against the oracle's round-off floor on the same cells
+    (oracle/floor.py), plus a weighted sanity bound on every cell."""
+    from oracle import floor as F
     m, o = gri
-    y0, P, dt = WL.gri_uneven(m.species, m.W, 192, seed=31)
+    y0, P, dt = WL.gri_uneven(m.species, m.W, 256, seed=31)
+    fl = F.roundoff_floor(o, P, dt, y0, nthreads=8)
     yg, sg, stg = m.integrate(y0, P, dt, lu_precision=lu_precision, refine_sweeps=1)
-    yo, so, sto = o.integrate(P, dt, y0, nthreads=8)
+    yo, so, sto = fl["y"], fl["status"], fl["stats"]
     assert (sg == 0).all() and (so == 0).all()
+    print("floor", F.summary(fl), "gpu", F.compare(yg, sg, stg, fl), flush=True)
+    F.check_against_floor(yg, sg, stg, fl)
     dT = np.abs(yg[:, 0] - yo[:, 0]) / yo[:, 0]
     assert dT.max() < 1e-5
     w = np.abs(yg - yo) / (1e-6 * np.abs(yo) + 1e-10)

@@ -1,0 +1,72 @@
+"""The match criterion of the integration parity tests, checked on the CPU.
+
+The stated bar (T and every mass fraction above 1e-10 within a relative 1e-6 of the oracle)
+is tested against the round-off floor of the same oracle (oracle/floor.py): the share of
+cells on which the CPU BDF agrees with itself when one input moves by one ulp. These tests
+pin the checker: the floor is a property of the method and the cells, the checker accepts
+runs that differ from the reference only by rounding, and it rejects runs with a planted
+mistake that is small against the tolerance."""
+import numpy as np
+import pytest
+
+import oracle as O
+from hpc_ode_amd import workloads as WL
+from oracle import floor as F
+
+
+@pytest.fixture(scope="module")
+def gri_floor(gri_path):
+    o = O.Oracle(gri_path)
+    y0, P, dt = WL.gri_plain(o.mech.species, np.array(o.mech.W), 256, seed=11)
+    return o, y0, P, dt, F.roundoff_floor(o, P, dt, y0, nthreads=8)
+
+
+def test_floor_is_a_property_of_the_cells(gri_floor):
+    """On the plain GRI workload about half of the cells change at the bar under a one-ulp
+    input change, and which half is set by the chemistry: cells below 1400 K (slow
+    induction) keep their answer, cells above 1800 K (ignition and NOx chemistry inside the
+    substep) mostly do not."""
+    o, y0, P, dt, fl = gri_floor
+    assert (fl["status"] == 0).all()
+    assert 0.3 < fl["fraction"] < 0.8, fl["fraction"]
+    assert fl["robust_keep_rate"] >= 0.97
+    T0 = y0[:, 0]
+    assert fl["robust"][T0 < 1400].mean() > 0.95
+    assert fl["robust"][T0 > 1800].mean() < 0.3
+
+
+def test_checker_accepts_rounding_differences(gri_floor):
+    """A run whose every input moved by one ulp, and the FP32 Newton matrix with one FP64
+    refinement sweep, meet the floor."""
+    o, y0, P, dt, fl = gri_floor
+    y, s, st = o.integrate(P, dt, np.nextafter(y0, 2.0), nthreads=8)
+    F.check_against_floor(y, s, st, fl)
+    y, s, st = o.integrate(P, dt, y0, lu_fp32=True, refine_sweeps=1, nthreads=8)
+    F.check_against_floor(y, s, st, fl)
+
+
+@pytest.mark.parametrize("mistake", ["rtol 1.5e-6", "pressure off by 1e-7", "FP32 LU without refinement"])
+def test_checker_rejects_small_mistakes(gri_floor, mistake):
+    o, y0, P, dt, fl = gri_floor
+    kw, Pm = {}, P
+    if mistake == "rtol 1.5e-6":
+        kw = dict(rtol=1.5e-6)
+    elif mistake == "pressure off by 1e-7":
+        Pm = P * (1.0 + 1e-7)
+    else:
+        kw = dict(lu_fp32=True, refine_sweeps=0)
+    y, s, st = o.integrate(Pm, dt, y0, nthreads=8, **kw)
+    with pytest.raises(AssertionError):
+        F.check_against_floor(y, s, st, fl)
+
+
+def test_h2_floor(h2_path):
+    """The H2/air CPU configuration (100 cells, dt 1e-3 s): the floor measured the same way;
+    every cell ignites and reaches equilibrium inside the substep."""
+    o = O.Oracle(h2_path)
+    y0, P, dt = WL.h2_air(o.mech.species, np.array(o.mech.W), 100, seed=12)
+    fl = F.roundoff_floor(o, P, dt, y0, nthreads=8)
+    assert (fl["status"] == 0).all()
+    assert 0.0 <= fl["fraction"] <= 1.0 and fl["robust_keep_rate"] >= 0.9
+    y, s, st = o.integrate(P, dt, np.nextafter(y0, 2.0), nthreads=8)
+    F.check_against_floor(y, s, st, fl)
