@@ -76,16 +76,16 @@ def test_substeps_continue_from_resident_state(mechs):
 def test_admission_against_hbm_budget(mechs):
     """A stream is charged the pool block that holds it: its rows, pressures and statuses
     (53 x 8 + 8 + 4 bytes per GRI cell, 256-byte aligned parts) rounded up to a power of two,
-    here 512 KiB for 600 cells. A budget of 1,000,000 bytes admits one such stream, not two."""
+    here 256 KiB for 600 cells. A budget of 400,000 bytes admits one such stream, not two."""
     gri = mechs[1]
-    srv = H.Server(mechs, hbm_budget_bytes=1_000_000)
+    srv = H.Server(mechs, hbm_budget_bytes=400_000)
     try:
         y, P = _state(gri, 1, 600, 1)
         a = srv.open_stream(1, y, P)
         assert a is not None
         assert srv.open_stream(1, y, P) is None
         used, budget = srv.hbm()
-        assert used == 1 << 19 and budget == 1_000_000
+        assert used == 1 << 18 and budget == 400_000
         srv.close_stream(a)
         assert srv.hbm()[0] == 0
         b = srv.open_stream(1, y, P)          # reuses the pooled block
