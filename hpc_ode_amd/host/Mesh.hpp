@@ -47,9 +47,11 @@ class PointBuilder {
  public:
   explicit PointBuilder(int nsp) : p_(Point(nsp)) {}
   operator Point() const { return p_; }
+  // Each field can be set once (std::logic_error otherwise). addTemperature rejects T < 0;
+  // addSpecies rejects a vector whose length differs from nsp, an entry above 1 or a sum
+  // outside (0.99, 1.01) (std::runtime_error, the reference's PointBuilder.cpp:8-70).
   PointBuilder& addTemperature(double T);
   PointBuilder& addEnthalpy(double h);
-  // Throws std::invalid_argument when the vector length differs from the point's nsp.
   PointBuilder& addSpecies(std::vector<double> Y);
   PointBuilder& addCoordinates(Coords c);
 

@@ -51,8 +51,27 @@ static void test_point_and_mesh() {
   CHECK(p.hasTemperature() && p.hasEnthalpy() && p.hasCoordinates() && p.hasSpecies() && p.isReady());
   Point bad = Point::create(3).addSpecies({0.2, 0.2, 0.6});
   CHECK(!bad.isReady());
-  CHECK(message_of<std::invalid_argument>([] { Point q = Point::create(3).addSpecies({1.0}); (void)q; }) !=
+  // the reference builder's checks (src/Mesh/PointBuilder.cpp:8-70)
+  CHECK(message_of<std::runtime_error>([] { Point q = Point::create(3).addSpecies({1.0}); (void)q; }) ==
+        "Vector size does not match the number of the species of the Point");
+  CHECK(message_of<std::runtime_error>([] { Point q = Point::create(3).addSpecies({0.2, 0.2, 0.5}); (void)q; }) ==
+        "Species vector breaks mass conservation law");
+  CHECK(message_of<std::runtime_error>([] { Point q = Point::create(3).addSpecies({1.5, -0.5, 0.0}); (void)q; }) ==
+        "Species vector breaks mass conservation law");
+  CHECK(message_of<std::runtime_error>([] { Point q = Point::create(3).addSpecies({0.2, 0.2, 0.605}); (void)q; }) ==
         "<no exception>");
+  CHECK(message_of<std::runtime_error>([] { Point q = Point::create(3).addTemperature(-1.0); (void)q; }) ==
+        "Temperature cannot be negative");
+  CHECK(message_of<std::logic_error>([] { Point q = Point::create(3).addTemperature(1.0).addTemperature(2.0); (void)q; }) ==
+        "Temperature already set in the build process");
+  CHECK(message_of<std::logic_error>([] { Point q = Point::create(3).addEnthalpy(1.0).addEnthalpy(2.0); (void)q; }) ==
+        "Enthalpy already set in the build process");
+  CHECK(message_of<std::logic_error>([] {
+          Point q = Point::create(3).addSpecies({0.2, 0.2, 0.6}).addSpecies({0.2, 0.2, 0.6});
+          (void)q;
+        }) == "Species vector already set in the build process");
+  CHECK(message_of<std::logic_error>([] { Point q = Point::create(3).addCoordinates({0, 0, 0}).addCoordinates({1, 1, 1}); (void)q; }) ==
+        "Coordinates already set in the build process");
 
   Mesh m;
   for (int i = 0; i < 3; ++i) m.addPoint(full_point(100 + i));
