@@ -142,6 +142,7 @@ struct Tabs {
   const uint16_t* pptr;    // [K+1]
   const uint16_t* slot_base;  // [R+1] (global)
   const uint64_t* jterm;      // [njl][64] (global)
+  const double* exptab;       // [32] 2^(j/32) for tab_exp
   int K, R, ngl, njl, npart, neffp, npdep, mol_ok;
 };
 
@@ -162,6 +163,7 @@ HPO_DI Tabs make_tabs(const uint8_t* lds, const uint8_t* glob, const DevMechLayo
   t.coef = reinterpret_cast<const double*>(lds + L.off_coef);
   t.gather = reinterpret_cast<const uint32_t*>(lds + L.off_gather);
   t.pptr = reinterpret_cast<const uint16_t*>(lds + L.off_part_ptr);
+  t.exptab = reinterpret_cast<const double*>(lds + L.off_exptab);
   t.slot_base = reinterpret_cast<const uint16_t*>(glob + L.off_slot_base);
   t.jterm = reinterpret_cast<const uint64_t*>(glob + L.off_jterm);
   t.K = L.K;
@@ -265,9 +267,10 @@ HPO_DI double cpow_int(double c, int nu) {
 }
 HPO_DI double dcpow_int(double c, int nu) { return nu == 1 ? 1.0 : (nu == 2 ? 2.0 * c : 3.0 * c * c); }
 
-HPO_DI double arrh(double lnA, double b, double th, bool constk, double lnT, double invT) {
-  return constk ? lnA : exp(lnA + b * lnT - th * invT);
+HPO_DI double arrh(double lnA, double b, double th, bool constk, double lnT, double invT, const double* et) {
+  return constk ? lnA : tab_exp(lnA + b * lnT - th * invT, et);
 }
+constexpr double kLn10 = 2.302585092994045684;
 
 // Balanced gather: out[k] = sum_r nu_kr src[r] for species k = lane (0 for lane >= K).
 // Each lane walks its share of the nonzeros, flushing partial sums per species run;
@@ -329,7 +332,7 @@ HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, i
   const RxnHdr h = t.hdr[r];
   const double* ar = t.arr + 3 * r;
   // straight-line for the elementary part, so several reactions per lane can be interleaved
-  double e = exp(ar[0] + ar[1] * s.lnT - ar[2] * s.invT);
+  double e = tab_exp(ar[0] + ar[1] * s.lnT - ar[2] * s.invT, t.exptab);
   double kf = (h.flags & 2) ? ar[0] : e;
   double fwd = kf, pc = 1.0, dG = 0.0, dnu = 0.0;
   if constexpr (MOL) {
@@ -354,7 +357,7 @@ HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, i
     }
   }
   // k_r = k_f / Kc, Kc = (P_atm/(R T))^dnu exp(-dG)
-  double rev = (h.flags & 1) ? kf * exp(dG - dnu * s.lnPfac) * pc : 0.0;
+  double rev = (h.flags & 1) ? kf * tab_exp(dG - dnu * s.lnPfac, t.exptab) * pc : 0.0;
   double pm = 1.0;
   if (PDEP && h.kind != kElementary) {
     const PdepRec& pr = t.pd[h.pidx];
@@ -375,18 +378,19 @@ HPO_DI double reaction_rate(const Tabs& t, const WaveLds& w, const SpState& s, i
     if (h.kind == kThreeBody) {
       pm = M;
     } else {
-      double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, h.flags & 8, s.lnT, s.invT);
+      double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, h.flags & 8, s.lnT, s.invT, t.exptab);
       double Pr = k0 * M / kf;
       double F = 1.0;
       if (h.kind == kTroe) {
-        double Fc = (1.0 - pr.troe_a) * exp(-s.T * pr.troe_iT3) + pr.troe_a * exp(-s.T * pr.troe_iT1) +
-                    ((h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0);
+        double Fc = (1.0 - pr.troe_a) * tab_exp(-s.T * pr.troe_iT3, t.exptab) +
+                    pr.troe_a * tab_exp(-s.T * pr.troe_iT1, t.exptab) +
+                    ((h.flags & 4) ? tab_exp(-pr.troe_T2 * s.invT, t.exptab) : 0.0);
         double logFc = fast_log10(fmax(Fc, 1.0e-300));
         double x = fast_log10(fmax(Pr, 1.0e-300));
         double A = x - 0.67 * logFc - 0.4;
         double B = 0.806 - 1.1762 * logFc - 0.14 * x;
         const double B2 = B * B;
-        F = exp10(logFc * B2 / (B2 + A * A));   // logFc / (1 + A^2/B^2)
+        F = tab_exp(kLn10 * (logFc * B2 / (B2 + A * A)), t.exptab);   // 10^(logFc / (1 + A^2/B^2))
       }
       pm = F * Pr / (1.0 + Pr);
     }
@@ -486,7 +490,7 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
     const RxnHdr h = t.hdr[r];
     const double* ar = t.arr + 3 * r;
     const bool constk = h.flags & 2;
-    double kf = arrh(ar[0], ar[1], ar[2], constk, s.lnT, s.invT);
+    double kf = arrh(ar[0], ar[1], ar[2], constk, s.lnT, s.invT, t.exptab);
     double dlnkf = constk ? 0.0 : (ar[1] + ar[2] * s.invT) * s.invT;
     double cr[3], cp_[3];
     double fwd = kf;
@@ -516,7 +520,7 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
         }
       dH = dHRT * RT;
       if (h.flags & 1) {
-        kr = kf * exp(dG - dnu * s.lnPfac);
+        kr = kf * tab_exp(dG - dnu * s.lnPfac, t.exptab);
         rev = kr;
 #pragma unroll
         for (int q = 0; q < 3; ++q)
@@ -538,15 +542,15 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
         dpm_dM = 1.0;
       } else {
         const bool constk0 = h.flags & 8;
-        double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, constk0, s.lnT, s.invT);
+        double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, constk0, s.lnT, s.invT, t.exptab);
         double dlnk0 = constk0 ? 0.0 : (pr.b0 + pr.theta0 * s.invT) * s.invT;
         double Pr = k0 * M / kf;
         double dPr_dT = Pr * (dlnk0 - dlnkf);
         double dPr_dM = k0 / kf;
         double qq = 1.0 / (1.0 + Pr);
         if (h.kind == kTroe) {
-          double e3 = exp(-s.T * pr.troe_iT3), e1 = exp(-s.T * pr.troe_iT1);
-          double e2 = (h.flags & 4) ? exp(-pr.troe_T2 * s.invT) : 0.0;
+          double e3 = tab_exp(-s.T * pr.troe_iT3, t.exptab), e1 = tab_exp(-s.T * pr.troe_iT1, t.exptab);
+          double e2 = (h.flags & 4) ? tab_exp(-pr.troe_T2 * s.invT, t.exptab) : 0.0;
           double Fc = (1.0 - pr.troe_a) * e3 + pr.troe_a * e1 + e2;
           double dFc = -(1.0 - pr.troe_a) * pr.troe_iT3 * e3 - pr.troe_a * pr.troe_iT1 * e1 +
                        ((h.flags & 4) ? pr.troe_T2 * s.invT * s.invT * e2 : 0.0);
@@ -556,7 +560,7 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
           double A = x - 0.67 * logFc - 0.4;
           double B = 0.806 - 1.1762 * logFc - 0.14 * x;
           double D = 1.0 + A * A / (B * B);
-          double F = exp10(logFc / D);
+          double F = tab_exp(kLn10 * (logFc / D), t.exptab);
           double dD_dA = 2.0 * A / (B * B), dD_dB = -2.0 * A * A / (B * B * B);
           double df_dx = prl ? 0.0 : -logFc / (D * D) * (dD_dA - 0.14 * dD_dB);
           double df_dc = 1.0 / D - logFc / (D * D) * (-0.67 * dD_dA - 1.1762 * dD_dB);

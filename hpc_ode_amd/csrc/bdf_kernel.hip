@@ -349,7 +349,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
       }
       S.etamax = 1.0;
       if (nef <= MXNEF1) {
-        double eta = 1.0 / (fast_pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+        double eta = 1.0 / (fast_pow(BIAS2 * dsm, 1.0 / L, t.exptab) + ADDON);
         eta = fmax(ETAMIN, eta);
         if (nef >= SMALL_NEF) eta = fmin(eta, ETAMXF);
         S.eta = eta;
@@ -402,7 +402,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
       S.hprime = h;
       S.eta = 1.0;
     } else {
-      double etaq = 1.0 / (fast_pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+      double etaq = 1.0 / (fast_pow(BIAS2 * dsm, 1.0 / L, t.exptab) + ADDON);
       double eta;
       if (qwait != 0) {
         eta = etaq;
@@ -412,7 +412,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
         double etaqm1 = 0.0, etaqp1 = 0.0;
         if (q > 1) {
           double ddn = wrms(pick(zn, q), ewt, n) * S.tq[1];
-          etaqm1 = 1.0 / (fast_pow(BIAS1 * ddn, 1.0 / q) + ADDON);
+          etaqm1 = 1.0 / (fast_pow(BIAS1 * ddn, 1.0 / q, t.exptab) + ADDON);
         }
         if (q != QMAX && S.saved_tq5 != 0.0) {
           double rr = h / S.tau[2], pw = 1.0;
@@ -420,7 +420,7 @@ __device__ void integrate_cell(const BdfArgs& A, const Tabs& t, const WaveLds& w
           double cquot = (S.tq[5] / S.saved_tq5) * pw;
           double tempv = -cquot * zn[QMAX] + acor;
           double dup = wrms(tempv, ewt, n) * S.tq[3];
-          etaqp1 = 1.0 / (fast_pow(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+          etaqp1 = 1.0 / (fast_pow(BIAS3 * dup, 1.0 / (L + 1), t.exptab) + ADDON);
         }
         double etam = fmax(etaqm1, fmax(etaq, etaqp1));
         if (etam < THRESH) {

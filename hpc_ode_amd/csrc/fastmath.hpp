@@ -58,8 +58,48 @@ HPO_HD double fast_log(double x) {
 
 HPO_HD double fast_log10(double x) { return fast_log(x) * 0.43429448190325182765; }
 
+// FP64 exponential by table lookup and a short polynomial (P. T. P. Tang, "Table-driven
+// implementation of the exponential function in IEEE floating-point arithmetic", ACM TOMS
+// 15(2), 1989): x = (32 m + j) ln2/32 + r with |r| <= ln2/64, exp(x) = 2^m 2^(j/32) exp(r),
+// exp(r) - 1 = r + r^2 (1/2 + r/6 + r^2/24 + r^3/120 + r^4/720) (Taylor; the first omitted
+// term, r^7/5040, is below 2^-60 of the result). ln2/32 is split Cody-Waite style: LN2_32_HI
+// has 32 significant bits, so k LN2_32_HI is exact for every |k| < 2^21, which covers the
+// finite range. The table holds 2^(j/32) correctly rounded (kExp2Tab32; on the device a copy
+// in LDS is passed in). Error: at most 2 ulp against the C library (tests/test_fastmath.py).
+// Results overflow to +inf above 709.78 and underflow through the subnormals to 0 below
+// -745.13, as the C library's do; NaN propagates. Infinite arguments are not handled (they
+// give NaN): the kernels pass finite arguments.
+constexpr double kExp2Tab32[32] = {
+    0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
+    0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
+    0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,
+    0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,
+    0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,
+    0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0};
+
+HPO_HD double tab_exp(double x, const double* tab) {
+  constexpr double INV_LN2_32 = 0x1.71547652b82fep+5;   // 32 / ln 2
+  constexpr double LN2_32_HI = 0x1.62e42ff000000p-6;
+  constexpr double LN2_32_LO = -0x1.718432a1b0e26p-40;
+  const double k = std::rint(x * INV_LN2_32);
+  const int ki = static_cast<int>(fmax(fmin(k, 2147483520.0), -2147483520.0));
+  double r = std::fma(-k, LN2_32_HI, x);
+  r = std::fma(-k, LN2_32_LO, r);
+  double p = 0x1.6c16c16c16c17p-10;           // 1/720
+  p = std::fma(p, r, 0x1.1111111111111p-7);   // 1/120
+  p = std::fma(p, r, 0x1.5555555555555p-5);   // 1/24
+  p = std::fma(p, r, 0x1.5555555555555p-3);   // 1/6
+  p = std::fma(p, r, 0.5);
+  const double q = std::fma(r * r, p, r);     // exp(r) - 1
+  const double t = tab[ki & 31];
+  return std::ldexp(std::fma(t, q, t), ki >> 5);
+}
+
 // x^y for x >= 0 as exp(y ln x); used for the step-size ratios of the BDF controller, where
 // x is a positive error-norm ratio and y = 1/L with L = 2..6.
 HPO_HD double fast_pow(double x, double y) { return std::exp(y * fast_log(x)); }
+HPO_HD double fast_pow(double x, double y, const double* exptab) { return tab_exp(y * fast_log(x), exptab); }
 
 }  // namespace hpo

@@ -1,4 +1,5 @@
 #include "mechanism.hpp"
+#include "fastmath.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -660,6 +661,8 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   L.off_part_ptr = off;
   off += align_to((K + 1) * 2, 16);
   off = align_to(off, 16);
+  L.off_exptab = off;
+  off += 32 * 8;
   L.lds_bytes = off;
   L.off_slot_base = off;
   off += align_to((R + 1) * 2, 16);
@@ -757,6 +760,7 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   for (size_t i = 0; i < coef.size(); ++i) ctab[i] = coef[i];
   std::memcpy(B + L.off_gather, gather.data(), gather.size() * 4);
   std::memcpy(B + L.off_part_ptr, part_ptr.data(), part_ptr.size() * 2);
+  std::memcpy(B + L.off_exptab, kExp2Tab32, sizeof(kExp2Tab32));
   std::memcpy(B + L.off_slot_base, slot_base.data(), slot_base.size() * 2);
   std::memcpy(B + L.off_jterm, jterm.data(), jterm.size() * 8);
   return blob;

@@ -106,6 +106,7 @@ struct DevMechLayout {
   int32_t off_gather;           // uint32[n_gather_lane][64]  r | nu<<12 | flush<<16 | part<<17
                                 //   (nu: net stoichiometric coefficient, 4-bit two's complement)
   int32_t off_part_ptr;         // uint16[K+1] partial-slot range per species
+  int32_t off_exptab;           // double[32]: 2^(j/32), the table of the kernels' exponential
   // global part
   int32_t off_slot_base;        // uint16[R+1]: first d(phi)/dC slot of each reaction
   int32_t off_jterm;            // uint64[n_jterm_lane][64]  cell | src<<16 | coef<<32 | last<<48

@@ -45,3 +45,19 @@ def test_special_values(results):
     assert results["logneg"] != results["logneg"]      # NaN
     assert results["lognan"] != results["lognan"]
     assert results["pow0"] == 0.0
+
+
+def test_exp_within_two_ulp(results):
+    """tab_exp, the exponential of the rate constants, equilibrium constants, Troe terms and
+    step-size powers, against the C library's exp."""
+    assert results["exp"] <= 2.0
+    assert results["exp_small"] <= 2.0
+    assert results["exp_subnormal"] <= 1.0      # in units of the smallest subnormal
+    assert results["tab_pow"] <= 8.0
+
+
+def test_exp_special_values(results):
+    assert results["exp0"] == 1.0
+    assert results["exp_big"] == float("inf") and results["exp_tiny"] == 0.0
+    assert results["exp_nan"] != results["exp_nan"]
+    assert 1.79e308 < results["exp_max"] < float("inf")
