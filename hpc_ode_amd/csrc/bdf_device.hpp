@@ -221,7 +221,7 @@ inline int64_t wave_scratch_doubles(const DevMechLayout& L) {
 // species-level state shared by the RHS and the Jacobian
 // ---------------------------------------------------------------------------
 struct SpState {
-  double T, lnT, invT, P, rho, Wbar, m, lnPfac;
+  double T, lnT, invT, P, rho, invrho, Wbar, m, lnPfac;
   double Yk, invW, W, cpR, hRT, sR, dcpR;  // per lane (species k = lane)
 };
 
@@ -248,8 +248,9 @@ HPO_DI void species_state(const Tabs& t, double y, double P, int lane_in, SpStat
   s.Yk = Yk;
   double S = yw + ylast * t.sp[(K - 1) * kSpStride];
   s.Wbar = 1.0 / S;
-  s.rho = P * s.Wbar / (kR * T);
-  s.m = P / (kR * T);
+  s.m = P * s.invT * (1.0 / kR);   // P / (R T)
+  s.rho = s.m * s.Wbar;
+  s.invrho = S / s.m;
   s.lnPfac = kLnPatmOverR - s.lnT;   // log(P_atm / (R T))
   const double* a = rec + 3 + (T <= rec[2] ? 0 : 7);
   s.cpR = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
@@ -451,9 +452,9 @@ HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane_
   const double Hk = s.hRT * kR * s.T;  // molar enthalpy
   double Q = wsum(lane < K ? Hk * wdot : 0.0);
   double cpmix = wsum(lane < K ? s.Yk * s.cpR * kR * s.invW : 0.0);
-  double fY = wdot * s.W / s.rho;
+  double fY = wdot * s.W * s.invrho;
   double f = shfl(fY, (lane + 63) & 63);
-  if (lane == 0) f = -Q / (s.rho * cpmix);
+  if (lane == 0) f = -Q * s.invrho / cpmix;
   if (lane >= n) f = 0.0;
   return f;
 }
@@ -859,7 +860,8 @@ HPO_DI bool gj_invert(LuReg<NS, float>& L, int lane_in) {
     float piv = rl(L.a[k], p);
     bool bad = (bal == 0ull) || !(fabsf(piv) > 0.f);
     singular = singular || bad;
-    float mult = bad ? 0.f : 1.f / piv;
+    // hardware reciprocal (1 ulp): the FP32 inverse is refined against the FP64 matrix
+    float mult = bad ? 0.f : __builtin_amdgcn_rcpf(piv);
     const bool isp = lane == p;
     if (!bad) {
       if (pos == k) pos = ppos;
