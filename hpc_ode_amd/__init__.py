@@ -77,6 +77,7 @@ class RequestInfo(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+E_INVALID = -1
 E_BUSY = -6
 E_TIMEOUT = -7
 
@@ -109,6 +110,8 @@ def lib():
         L.hpo_batch_get.argtypes = [vp, C.c_int64, C.c_int64, _dp, C.c_int64, _ip, _ip]
         L.hpo_integrate_device.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp, vp,
                                            C.POINTER(Opts), vp, C.POINTER(Timing)]
+        L.hpo_integrate_device_noalloc.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp,
+                                                   C.POINTER(Opts), vp]
         L.hpo_eval_rhs.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
         L.hpo_eval_jac.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
         L.hpo_lu_solve.argtypes = [C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip]

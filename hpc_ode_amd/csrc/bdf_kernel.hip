@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <float.h>
 
+#include <mutex>
+
 #include "bdf_device.hpp"
 #include "kernels.hpp"
 
@@ -899,6 +901,32 @@ __global__ __launch_bounds__(64) void reset_counter(uint32_t* q) {
 hipError_t launch_reset_counter(uint32_t* q, hipStream_t st) {
   hipLaunchKernelGGL(dev::reset_counter, dim3(1), dim3(64), 0, st, q);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the no-allocation entry point (hpo_integrate_device_noalloc): the
+// work-queue counter in its first 256 bytes, then the per-wave slots. One copy per GPU, held
+// until the process ends. In host code `scratch` names a host-side stand-in, not device
+// memory: only the address hipGetSymbolAddress returns for the current device is used.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kNoallocScratchBytes / sizeof(double)];
+
+namespace {
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoallocMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+  if (device < 0 || device >= kNoallocMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_ptr[device] = p;
+  }
+  *ptr = g_noalloc_ptr[device];
+  return hipSuccess;
 }
 
 hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st) {

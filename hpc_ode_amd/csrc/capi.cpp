@@ -431,6 +431,75 @@ int hpo_integrate_device(const hpo_mech* mc, int64_t ncells, const double* d_y_i
   }
 }
 
+int hpo_integrate_device_noalloc(const hpo_mech* mc, int64_t ncells, const double* d_y_in, double* d_y_out,
+                                 int64_t ld, const double* d_P, const double* d_dt, int32_t* d_status,
+                                 int32_t* d_stats, const int32_t* d_order, const hpo_opts* o, void* stream) {
+  if (!mc || !d_y_in || !d_y_out || !d_P || !d_dt || !d_status) return fail(HPO_E_INVALID, "null argument");
+  hpo_opts def;
+  hpo_opts_default(&def);
+  const hpo_opts& opt = o ? *o : def;
+  try {
+    check_opts(opt);
+    const hpo_mech* m = mc;
+    const int ns = m->blob.lay.K;
+    if (!hpo::kernel_supports(ns))
+      throw std::invalid_argument("no kernel instantiated for a state of " + std::to_string(ns) + " variables");
+    if (ncells < 0 || ncells > 0x7fffffffLL) throw std::invalid_argument("ncells out of range");
+    if (ld < ns) throw std::invalid_argument("leading dimension smaller than the state");
+    int device = -1;
+    HPO_HIP(hipGetDevice(&device));
+    if (!m->d_blob || m->device != device)
+      throw std::invalid_argument(
+          "the mechanism's tables are not on the current device: run one allocating call first "
+          "(hpo_integrate_device with ncells = 0 uploads them)");
+    void* base = nullptr;
+    HPO_HIP(hpo::noalloc_scratch(device, &base));
+    const int wpb = opt.waves_per_block > 0 ? opt.waves_per_block : 4;
+    const int block = 64 * wpb;
+    const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) +
+                       static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
+    const int variant = hpo::bdf_variant(opt.lu_precision, opt.lu_layout);
+    int occ = 0;
+    HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_kernel_fn(ns, variant, 0), block, lds));
+    if (occ < 1) throw std::invalid_argument("kernel does not fit on a CU with this configuration");
+    const int bpc = opt.blocks_per_cu > 0 ? std::min(opt.blocks_per_cu, occ) : occ;
+    int64_t grid = static_cast<int64_t>(m->n_cu) * bpc;
+    grid = std::max<int64_t>(std::min<int64_t>(grid, (ncells + wpb - 1) / wpb), 1);
+    const int64_t stride = m->scratch_stride;
+    const size_t need = 256 + static_cast<size_t>(grid * wpb * stride) * sizeof(double);
+    if (need > hpo::kNoallocScratchBytes)
+      throw std::invalid_argument("launch needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                                  std::to_string(hpo::kNoallocScratchBytes) +
+                                  " bytes of the no-allocation entry point (fewer blocks_per_cu fit)");
+    hpo::BdfArgs A{};
+    A.mech = m->d_blob;
+    A.lay = m->blob.lay;
+    A.y_in = d_y_in;
+    A.y_out = d_y_out;
+    A.P = d_P;
+    A.dt = d_dt;
+    A.status = d_status;
+    A.stats = d_stats;
+    A.order = d_order;
+    A.queue = static_cast<uint32_t*>(base);
+    A.scratch = reinterpret_cast<double*>(static_cast<uint8_t*>(base) + 256);
+    A.scratch_stride = stride;
+    A.ncells = ncells;
+    A.ld = ld;
+    A.rtol = opt.rtol;
+    A.atol = opt.atol;
+    A.maxsteps = opt.maxsteps;
+    A.refine_sweeps = opt.refine_sweeps;
+    A.prof = nullptr;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HPO_HIP(hpo::launch_reset_counter(A.queue, st));
+    if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
+    return HPO_OK;
+  } catch (...) {
+    return map_exceptions();
+  }
+}
+
 static int eval_common(const hpo_mech* mc, int64_t n, const double* y, const double* P, double* out, int mode) {
   if (!mc || !y || !P || !out || n < 0) return fail(HPO_E_INVALID, "bad argument");
   if (n == 0) return HPO_OK;

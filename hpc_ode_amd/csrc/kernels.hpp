@@ -69,6 +69,13 @@ int64_t wave_scratch_doubles_host(const DevMechLayout& L);
 const void* bdf_kernel_fn(int ns, int variant, int prof = 0);
 hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st);
 hipError_t launch_reset_counter(uint32_t* q, hipStream_t st);
+// The no-allocation entry point's fixed scratch: 192 MiB of static device memory per GPU (the
+// counter in the first 256 bytes, then the wave slots), enough for a whole-device launch of a
+// 53-variable FP32 kernel at two waves per SIMD. noalloc_scratch returns its address on the
+// current device (hipGetSymbolAddress on the first call for that device, cached after).
+constexpr size_t kNoallocScratchBytes = size_t(192) << 20;
+constexpr int kNoallocMaxDevices = 64;
+hipError_t noalloc_scratch(int device, void** ptr);
 inline int bdf_variant(int lu_precision, int lu_layout) { return lu_precision == 0 ? 0 : (lu_layout == 0 ? 1 : 2); }
 hipError_t launch_eval(int ns, const EvalArgs& A, int grid, int block, size_t shmem, hipStream_t st);
 hipError_t launch_lu(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st);

@@ -120,6 +120,20 @@ int hpo_integrate_device(const hpo_mech *m, int64_t ncells, const double *d_y_in
                          int64_t ld, const double *d_P, const double *d_dt, int32_t *d_status,
                          int32_t *d_stats, const int32_t *d_order, int64_t *d_prof, const hpo_opts *o,
                          void *hip_stream, hpo_timing *timing);
+/* hpo_integrate_device for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. It calls no
+ * hipMalloc, hipFree or allocating routine: the work-queue counter and the per-wave scratch come
+ * from a fixed static device array of 192 MiB per GPU, held until the process ends, enough for a
+ * whole-device launch of the 53-variable GRI-3.0 kernels. A launch that would need more is
+ * rejected (HPO_E_INVALID, the message gives the bytes). The mechanism's tables must already be
+ * on the current device (any allocating integration call uploads them; hpo_integrate_device with
+ * ncells = 0 does nothing else). Every call on one GPU shares that GPU's array and clears its
+ * counter with a kernel on hip_stream: calls that overlap on one GPU race, so order them on one
+ * stream. It returns after the launch (no timing, no profile). */
+int hpo_integrate_device_noalloc(const hpo_mech *m, int64_t ncells, const double *d_y_in, double *d_y_out,
+                                 int64_t ld, const double *d_P, const double *d_dt, int32_t *d_status,
+                                 int32_t *d_stats, const int32_t *d_order, const hpo_opts *o,
+                                 void *hip_stream);
 
 /* ---- autotuner for uneven workloads (persistent kernel + work queue) ----
  * No counterpart in the reference (its GPU path integrates one batch with shared steps,
