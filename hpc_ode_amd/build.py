@@ -90,6 +90,10 @@ def build_host(verbose: bool = False) -> str:
     exe_link = ["-L", HERE, "-lhpc_ode_host", "-lhpc_ode", "-Wl,-rpath,$ORIGIN/..", *ROCM_LINK]
     _run([GXX, *HOST_CXX, "-o", os.path.join(BIN, "hpo_integrate_csv"), os.path.join(HOST_DIR, "hpo_integrate_csv.cpp"),
           *exe_link], "building hpo_integrate_csv", verbose)
+    # the reference application's entry point (config.yaml in, execution folder out)
+    _run([GXX, *HOST_CXX, "-I", CSRC, "-o", os.path.join(BIN, "ODEApplication"),
+          os.path.join(HOST_DIR, "ODEApplication.cpp"), os.path.join(CSRC, "yaml_lite.cpp"), *exe_link],
+         "building ODEApplication", verbose)
     test_src = os.path.join(ROOT, "tests", "cpp", "test_host.cpp")
     if os.path.exists(test_src):
         _run([GXX, *HOST_CXX, "-o", os.path.join(BIN, "test_host"), test_src, *exe_link], "building test_host", verbose)

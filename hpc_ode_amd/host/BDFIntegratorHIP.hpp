@@ -3,8 +3,9 @@
 // error control, on one MI355X, through the C ABI of include/hpc_ode.h.
 //
 // Differences a caller can see, all deliberate:
-//  * The mechanism is read from its YAML file at init (the reference compiles pyJac code for
-//    one mechanism); any mechanism of up to 64 species with the supported rate types works.
+//  * The mechanism is read from its YAML or CTML file at init (the reference compiles pyJac code
+//    for one mechanism). Kernels are instantiated for states of 9, 10 and 53 variables
+//    (h2_li_2004, h2o2, GRI-3.0 and other mechanisms of those sizes); init throws for others.
 //  * Device memory is owned from init to clean, so repeated integrate calls (substeps of a
 //    CFD run) reuse it; the reference GPU path allocates per call (CVodeIntegratorGPU.cpp:57,121).
 //  * The per-cell solver return code is kept (cellStatus()) and failed cells are logged; the

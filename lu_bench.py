@@ -1,6 +1,6 @@
 """Times the batched dense solve in its two layouts on GRI-3.0 Newton matrices.
 
-    python -m hpc_ode_amd.lu_bench [--matrices N] [--repeat R]
+    python lu_bench.py [--matrices N] [--repeat R]
 
 Layout 0 is one wave per matrix (the BDF kernel's register LU, FP64 and the FP32 inverse);
 layout 1 is one 256-thread workgroup per matrix with the augmented matrix in LDS (FP64). The
@@ -26,7 +26,7 @@ def main():
     import hpc_ode_amd as H
     from hpc_ode_amd import workloads as WL
 
-    mech = H.Mechanism(os.path.join(HERE, "mechanisms", "gri30.yaml"))
+    mech = H.Mechanism(os.path.join(HERE, "hpc_ode_amd", "mechanisms", "gri30.yaml"))
     rng = np.random.default_rng(5)
     y, P, _ = WL.gri_plain(mech.species, mech.W, a.states, seed=5)
     J = mech.jac(y, P)
