@@ -35,7 +35,7 @@ class Opts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("maxsteps", C.c_int32),
                 ("lu_precision", C.c_int32), ("refine_sweeps", C.c_int32),
                 ("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("sort_cells", C.c_int32), ("lu_layout", C.c_int32), ("reserved", C.c_int32)]
+                ("sort_cells", C.c_int32), ("lu_layout", C.c_int32), ("cell_layout", C.c_int32)]
 
 
 class Timing(C.Structure):
@@ -53,13 +53,13 @@ class TuneReport(C.Structure):
     _fields_ = [("n_candidates", C.c_int32), ("best", C.c_int32),
                 ("waves_per_block", C.c_int32 * TUNE_MAX), ("blocks_per_cu", C.c_int32 * TUNE_MAX),
                 ("order_mode", C.c_int32 * TUNE_MAX), ("lu_layout", C.c_int32 * TUNE_MAX),
-                ("sample_ms", C.c_double * TUNE_MAX),
+                ("cell_layout", C.c_int32 * TUNE_MAX), ("sample_ms", C.c_double * TUNE_MAX),
                 ("sample_cells", C.c_int64), ("tune_ms", C.c_double)]
 
     def candidates(self):
         return [dict(waves_per_block=self.waves_per_block[i], blocks_per_cu=self.blocks_per_cu[i],
                      order=ORDER_MODES[self.order_mode[i]], lu_layout=LU_LAYOUTS[self.lu_layout[i]],
-                     sample_ms=self.sample_ms[i])
+                     cell_layout=CELL_LAYOUTS[self.cell_layout[i]], sample_ms=self.sample_ms[i])
                 for i in range(self.n_candidates)]
 
 
@@ -114,6 +114,7 @@ def lib():
                                                    C.POINTER(Opts), vp]
         L.hpo_eval_rhs.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
         L.hpo_eval_jac.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
+        L.hpo_eval_layout.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp]
         L.hpo_lu_solve.argtypes = [C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip]
         L.hpo_lu_solve_layout.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip,
                                           C.POINTER(C.c_float)]
@@ -144,14 +145,18 @@ def _p(a, t=_dp):
     return a.ctypes.data_as(t)
 
 
+CELL_LAYOUTS = {0: "auto", 1: "wave", 2: "block"}
+
+
 def opts(rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0, refine_sweeps=1,
-         waves_per_block=0, blocks_per_cu=0, sort_cells=0, lu_layout=0) -> Opts:
+         waves_per_block=0, blocks_per_cu=0, sort_cells=0, lu_layout=0, cell_layout=0) -> Opts:
     o = Opts()
     lib().hpo_opts_default(C.byref(o))
     o.rtol, o.atol, o.maxsteps = rtol, atol, maxsteps
     o.lu_precision, o.refine_sweeps = lu_precision, refine_sweeps
     o.waves_per_block, o.blocks_per_cu, o.sort_cells = waves_per_block, blocks_per_cu, sort_cells
     o.lu_layout = lu_layout
+    o.cell_layout = cell_layout
     return o
 
 
@@ -196,26 +201,34 @@ class Mechanism:
         return dict(nasa=nasa, tmid=tmid, arr=arr, kind=kind, rev=rev)
 
     # --- component evaluations (GPU) -------------------------------------
-    def rhs(self, y, P):
+    def rhs(self, y, P, cell_layout=0):
+        """dydt on the GPU; cell_layout as in integrate (0 auto, 1 wave, 2 block)."""
         y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
         n = y.shape[0]
         P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
         f = np.zeros_like(y)
-        _check(lib().hpo_eval_rhs(self._h, n, _p(y), _p(P), _p(f)))
+        if cell_layout == 0:
+            _check(lib().hpo_eval_rhs(self._h, n, _p(y), _p(P), _p(f)))
+        else:
+            _check(lib().hpo_eval_layout(self._h, cell_layout, 0, n, _p(y), _p(P), _p(f)))
         return f
 
-    def jac(self, y, P):
+    def jac(self, y, P, cell_layout=0):
+        """d dydt / dy on the GPU, [n, n_state, n_state]; cell_layout as in rhs."""
         y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
         n = y.shape[0]
         P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
         J = np.zeros((n, self.n_state, self.n_state))
-        _check(lib().hpo_eval_jac(self._h, n, _p(y), _p(P), _p(J)))
+        if cell_layout == 0:
+            _check(lib().hpo_eval_jac(self._h, n, _p(y), _p(P), _p(J)))
+        else:
+            _check(lib().hpo_eval_layout(self._h, cell_layout, 1, n, _p(y), _p(P), _p(J)))
         return J
 
     # --- integration ------------------------------------------------------
     def integrate(self, y, P, dt, rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0,
                   refine_sweeps=1, waves_per_block=0, blocks_per_cu=0, sort_cells=0,
-                  return_timing=False, lu_layout=0):
+                  return_timing=False, lu_layout=0, cell_layout=0):
         """Integrates every row of y ([ncells, n_state]) from t=0 to dt on the GPU.
         Returns (y_out, status, stats[, timing])."""
         y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
@@ -223,7 +236,7 @@ class Mechanism:
         b = Batch(self, n)
         b.put(y, P, dt)
         t = b.integrate(n, opts(rtol, atol, maxsteps, lu_precision, refine_sweeps, waves_per_block,
-                                blocks_per_cu, sort_cells, lu_layout))
+                                blocks_per_cu, sort_cells, lu_layout, cell_layout))
         yo, st, stats = b.get(n)
         return (yo, st, stats, t) if return_timing else (yo, st, stats)
 
@@ -344,7 +357,8 @@ class Server:
 
 def lu_solve(M, b, fp32=False, layout=None, timing=False):
     """Batched LU + solve (parity tests). layout None: hpo_lu_solve, the BDF kernel's register
-    code; 0 / 1: hpo_lu_solve_layout with one wave / one 256-thread workgroup per matrix.
+    code; 0 / 1: hpo_lu_solve_layout with one wave / one 256-thread workgroup per matrix;
+    2: the block-per-cell kernel's Gauss-Jordan code (ns 53 or 100).
     timing=True also returns the launch's device time in ms (layouts 0 and 1)."""
     M = np.ascontiguousarray(M, np.float64)
     b = np.ascontiguousarray(b, np.float64)

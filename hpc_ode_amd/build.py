@@ -42,6 +42,7 @@ SOURCES = [
     ("capi.cpp", HOST_FLAGS),
     ("server.cpp", HOST_FLAGS),
     ("bdf_kernel.hip", HIP_FLAGS),
+    ("block_kernel.hip", HIP_FLAGS),
     ("lu_block.hip", HIP_FLAGS),
 ]
 

@@ -1,0 +1,777 @@
+// Block-per-cell persistent BDF kernel for gfx950: one 256-thread workgroup (four waves)
+// integrates one cell, thread t owning state component t, so mechanisms with more species
+// than a wave has lanes (nDodecane_Reitz, 100 species) run, and for smaller ones the
+// reactions and the Newton matrix spread over four waves instead of one.
+//
+// The controller is the one of the wave kernel (bdf_cell.hpp); this file provides the
+// workgroup versions of its operations:
+//  * reductions: a wave sum (DPP) per wave, then the four wave partials through LDS, added in
+//    a fixed order so every thread holds the same value;
+//  * RHS: species thermo on threads 0..K-1, reactions strided over the 256 threads, species
+//    production rates gathered per species from the compressed row list (fixed order);
+//  * Jacobian: per reaction the derivative slots, per species row the owner-computed
+//    d(wdot)/dC row, per column the enthalpy-weighted sum, then the chain rule to (T, Y) with
+//    one Jacobian row per thread in the workgroup's scratch;
+//  * Newton matrix: Gauss-Jordan inverse with LAPACK-style partial pivoting (largest candidate,
+//    lowest row on ties), one register row per thread, the pivot row broadcast through LDS;
+//    a solve is a matrix-vector product. FP64, or FP32 with FP64 residual refinement sweeps.
+#include <hip/hip_runtime.h>
+
+#include "bdf_cell.hpp"
+
+#ifndef HPO_BLK_WAVES_PER_EU_FP64
+#define HPO_BLK_WAVES_PER_EU_FP64 1
+#endif
+#ifndef HPO_BLK_WAVES_PER_EU_FP32
+#define HPO_BLK_WAVES_PER_EU_FP32 1
+#endif
+
+namespace hpo {
+namespace dev {
+
+constexpr int kBlkWaves = 4;
+constexpr int kBlkThreads = 64 * kBlkWaves;
+
+struct BlkTabs {
+  const double* sp;        // [K][kSpStride]
+  const BlkRxn* rxn;       // [R]
+  const double* arr;       // [R][3]
+  const PdepRec* pd;
+  const double* effv;
+  const uint8_t* effs;
+  const uint16_t* csr_ptr; // [K+1]
+  const uint32_t* csr;
+  const double* exptab;
+  const uint8_t* slot_sp;  // global
+  const uint32_t* col_ptr; // global
+  const uint32_t* col;     // global
+  int K, R;
+};
+
+HPO_DI BlkTabs make_blk_tabs(const uint8_t* lds, const uint8_t* glob, const BlkLayout& L) {
+  BlkTabs t;
+  t.sp = reinterpret_cast<const double*>(lds + L.off_species);
+  t.rxn = reinterpret_cast<const BlkRxn*>(lds + L.off_rxn);
+  t.arr = reinterpret_cast<const double*>(lds + L.off_arr);
+  t.pd = reinterpret_cast<const PdepRec*>(lds + L.off_pdep);
+  t.effv = reinterpret_cast<const double*>(lds + L.off_eff_val);
+  t.effs = lds + L.off_eff_sp;
+  t.csr_ptr = reinterpret_cast<const uint16_t*>(lds + L.off_csr_ptr);
+  t.csr = reinterpret_cast<const uint32_t*>(lds + L.off_csr);
+  t.exptab = reinterpret_cast<const double*>(lds + L.off_exptab);
+  t.slot_sp = glob + L.off_slot_sp;
+  t.col_ptr = reinterpret_cast<const uint32_t*>(glob + L.off_col_ptr);
+  t.col = reinterpret_cast<const uint32_t*>(glob + L.off_col);
+  t.K = L.K;
+  t.R = L.R;
+  return t;
+}
+
+// Workgroup LDS work area (after the tables). Sizes: NB = kBlkThreads.
+struct BlkLds {
+  double* y;      // [NB]   state broadcast
+  double* vec;    // [NB]   vector broadcast (solves, RHS result)
+  double* vec2;   // [NB]
+  double* C;      // [K+1]  concentrations (index K: the dummy species, C = 1)
+  double* g;      // [K+1]  G/RT (dummy: 0)
+  double* hRT;    // [K+1]  H/RT (dummy: 0)
+  double* cp;     // [K+1]  mass cp
+  double* phi;    // [R]    net rates
+  double* sp4;    // [5][K] Jacobian per-species values: wdot, JT, Jm, sigma, dQ/dC
+  double* red;    // [8 * 8] reduction partials
+  double* prow;   // [NB]   pivot row
+  int* perm;      // [NB]   pivot row of each Gauss-Jordan step
+  int* iscal;     // [16]   pivot keys per wave, work-queue cell
+};
+__host__ __device__ inline int blk_lds_doubles(int K, int R) {
+  return 3 * kBlkThreads + 4 * (K + 2) + ((R + 1) & ~1) + 5 * ((K + 1) & ~1) + 64 + kBlkThreads +
+         kBlkThreads / 2 + 8;
+}
+HPO_DI BlkLds make_blk_lds(double* base, int K, int R) {
+  BlkLds w;
+  double* p = base;
+  w.y = p; p += kBlkThreads;
+  w.vec = p; p += kBlkThreads;
+  w.vec2 = p; p += kBlkThreads;
+  w.C = p; p += K + 2;
+  w.g = p; p += K + 2;
+  w.hRT = p; p += K + 2;
+  w.cp = p; p += K + 2;
+  w.phi = p; p += (R + 1) & ~1;
+  w.sp4 = p; p += 5 * ((K + 1) & ~1);
+  w.red = p; p += 64;
+  w.prow = p; p += kBlkThreads;
+  w.perm = reinterpret_cast<int*>(p); p += kBlkThreads / 2;
+  w.iscal = reinterpret_cast<int*>(p);
+  return w;
+}
+
+// Per-workgroup scratch (doubles): d(phi)/dC slots, per-reaction values, d(wdot)/dC rows,
+// the Jacobian rows of the state.
+struct BlkScratch {
+  double *dphi, *gphi, *gdT, *gdm, *dH, *JC, *J;
+};
+__host__ __device__ inline int64_t blk_scratch_doubles(const BlkLayout& L, int NS) {
+  int64_t o = ((L.n_slots + 1) & ~1) + 4 * ((L.R + 1) & ~1) + static_cast<int64_t>(L.K) * L.K + static_cast<int64_t>(NS) * NS;
+  return (o + 31) & ~int64_t(31);
+}
+HPO_DI BlkScratch make_blk_scratch(double* base, const BlkLayout& L) {
+  BlkScratch s;
+  double* p = base;
+  s.dphi = p; p += (L.n_slots + 1) & ~1;
+  s.gphi = p; p += (L.R + 1) & ~1;
+  s.gdT = p; p += (L.R + 1) & ~1;
+  s.gdm = p; p += (L.R + 1) & ~1;
+  s.dH = p; p += (L.R + 1) & ~1;
+  s.JC = p; p += static_cast<int64_t>(L.K) * L.K;
+  s.J = p;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// workgroup reductions: identical result in every thread
+// ---------------------------------------------------------------------------
+template <int N>
+HPO_DI void bsum(double (&v)[N], double* red, int tid) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = wsum(v[j]);
+  __syncthreads();   // the previous reduction's readers are done with red
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) red[(tid >> 6) * 8 + j] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = (red[j] + red[8 + j]) + (red[16 + j] + red[24 + j]);
+}
+HPO_DI double bsum1(double v, double* red, int tid) {
+  double a[1] = {v};
+  bsum<1>(a, red, tid);
+  return a[0];
+}
+HPO_DI double bmax(double v, double* red, int tid) {
+  v = wmax(v);
+  __syncthreads();
+  if ((tid & 63) == 0) red[(tid >> 6) * 8] = v;
+  __syncthreads();
+  return fmax(fmax(red[0], red[8]), fmax(red[16], red[24]));
+}
+HPO_DI unsigned bmax_u(unsigned v, int* ired, int tid) {
+  v = wmax_u(v);
+  __syncthreads();
+  if ((tid & 63) == 0) ired[tid >> 6] = static_cast<int>(v);
+  __syncthreads();
+  const unsigned a = static_cast<unsigned>(ired[0]), b = static_cast<unsigned>(ired[1]);
+  const unsigned c = static_cast<unsigned>(ired[2]), d = static_cast<unsigned>(ired[3]);
+  return max(max(a, b), max(c, d));
+}
+
+// ---------------------------------------------------------------------------
+// species state on threads 0..K-1 (species k = thread k) from the broadcast state
+// ---------------------------------------------------------------------------
+HPO_DI void blk_species_state(const BlkTabs& t, const BlkLds& w, double P, int tid, SpState& s) {
+  const int K = t.K;
+  const double T = w.y[0];
+  s.T = T;
+  s.P = P;
+  s.invT = 1.0 / T;
+  s.lnT = fast_log(T);
+  const int k = tid;
+  double Yk = (k < K - 1) ? w.y[k + 1] : 0.0;
+  const double* rec = t.sp + (k < K ? k : K - 1) * kSpStride;
+  s.invW = rec[0];
+  s.W = rec[1];
+  double v[2] = {(k < K - 1) ? Yk : 0.0, (k < K - 1) ? Yk * s.invW : 0.0};
+  bsum<2>(v, w.red, tid);
+  const double ylast = 1.0 - v[0];
+  if (k == K - 1) Yk = ylast;
+  s.Yk = Yk;
+  const double S = v[1] + ylast * t.sp[(K - 1) * kSpStride];
+  s.Wbar = 1.0 / S;
+  s.m = P * s.invT * (1.0 / kR);
+  s.rho = s.m * s.Wbar;
+  s.invrho = S / s.m;
+  s.lnPfac = kLnPatmOverR - s.lnT;
+  const double* a = rec + 3 + (T <= rec[2] ? 0 : 7);
+  s.cpR = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
+  s.hRT = a[0] + T * (a[1] * 0.5 + T * (a[2] * (1.0 / 3.0) + T * (a[3] * 0.25 + T * a[4] * 0.2))) + a[5] * s.invT;
+  s.sR = a[0] * s.lnT + T * (a[1] + T * (a[2] * 0.5 + T * (a[3] * (1.0 / 3.0) + T * a[4] * 0.25))) + a[6];
+  s.dcpR = a[1] + T * (2.0 * a[2] + T * (3.0 * a[3] + T * 4.0 * a[4]));
+}
+
+// Broadcasts the state, evaluates the species state and writes C, g, H/RT, cp (and the dummy
+// species K) to LDS. Returns with those visible to the whole workgroup.
+HPO_DI void blk_prepare(const BlkTabs& t, const BlkLds& w, double y, double P, int tid, int ns, SpState& s) {
+  __syncthreads();   // previous readers of y / C / g are done
+  w.y[tid] = (tid < ns) ? y : 0.0;
+  __syncthreads();
+  blk_species_state(t, w, P, tid, s);
+  const int K = t.K;
+  if (tid < K) {
+    w.C[tid] = s.rho * s.Yk * s.invW;
+    w.g[tid] = s.hRT - s.sR;
+    w.hRT[tid] = s.hRT;
+    w.cp[tid] = s.cpR * kR * s.invW;
+  }
+  if (tid == K) {
+    w.C[K] = 1.0;
+    w.g[K] = 0.0;
+    w.hRT[K] = 0.0;
+    w.cp[K] = 0.0;
+  }
+  __syncthreads();
+}
+
+// Third-body concentration and falloff of a pressure-dependent reaction (rxn_rates_pres_mod.c):
+// pm, and d pm / d[M], d pm / dT at fixed [M] when DER.
+template <bool DER>
+HPO_DI void blk_pdep(const BlkTabs& t, const BlkLds& w, const SpState& s, const BlkRxn& h, double kf, double dlnkf,
+                     double& pm, double& M, double& dpm_dM, double& dpm_dT) {
+  const PdepRec& pr = t.pd[h.pidx];
+  M = pr.eff_base * s.m;
+  for (int e = pr.eff_begin; e < pr.eff_end; ++e) M += t.effv[e] * w.C[t.effs[e]];
+  dpm_dM = 0.0;
+  dpm_dT = 0.0;
+  if (h.kind == kThreeBody) {
+    pm = M;
+    dpm_dM = 1.0;
+    return;
+  }
+  const bool constk0 = h.flags & 8;
+  const double k0 = arrh(pr.lnA0, pr.b0, pr.theta0, constk0, s.lnT, s.invT, t.exptab);
+  const double dlnk0 = constk0 ? 0.0 : (pr.b0 + pr.theta0 * s.invT) * s.invT;
+  const double Pr = k0 * M / kf;
+  const double qq = 1.0 / (1.0 + Pr);
+  if (h.kind == kTroe) {
+    const double e3 = tab_exp(-s.T * pr.troe_iT3, t.exptab), e1 = tab_exp(-s.T * pr.troe_iT1, t.exptab);
+    const double e2 = (h.flags & 4) ? tab_exp(-pr.troe_T2 * s.invT, t.exptab) : 0.0;
+    const double Fc = (1.0 - pr.troe_a) * e3 + pr.troe_a * e1 + e2;
+    const double logFc = fast_log10(fmax(Fc, 1.0e-300));
+    const double x = fast_log10(fmax(Pr, 1.0e-300));
+    const double A = x - 0.67 * logFc - 0.4;
+    const double B = 0.806 - 1.1762 * logFc - 0.14 * x;
+    const double D = 1.0 + A * A / (B * B);
+    const double F = tab_exp(kLn10 * (logFc / D), t.exptab);
+    pm = F * Pr * qq;
+    if (DER) {
+      const double dFc = -(1.0 - pr.troe_a) * pr.troe_iT3 * e3 - pr.troe_a * pr.troe_iT1 * e1 +
+                         ((h.flags & 4) ? pr.troe_T2 * s.invT * s.invT * e2 : 0.0);
+      const bool fcl = !(Fc > 1.0e-300), prl = !(Pr > 1.0e-300);
+      const double dD_dA = 2.0 * A / (B * B), dD_dB = -2.0 * A * A / (B * B * B);
+      const double df_dx = prl ? 0.0 : -logFc / (D * D) * (dD_dA - 0.14 * dD_dB);
+      const double df_dc = 1.0 / D - logFc / (D * D) * (-0.67 * dD_dA - 1.1762 * dD_dB);
+      const double dc_dT = fcl ? 0.0 : dFc / (Fc * kLn10);
+      const double dF_dT = F * kLn10 * df_dc * dc_dT;
+      const double dpm_dPr = F * qq * qq + F * df_dx * qq;
+      dpm_dM = dpm_dPr * (k0 / kf);
+      dpm_dT = dpm_dPr * Pr * (dlnk0 - dlnkf) + Pr * qq * dF_dT;
+    }
+  } else {
+    pm = Pr * qq;
+    if (DER) {
+      dpm_dM = qq * qq * (k0 / kf);
+      dpm_dT = qq * qq * Pr * (dlnk0 - dlnkf);
+    }
+  }
+}
+
+// Net rate (fwd - rev) * pm of reaction r.
+HPO_DI double blk_rate(const BlkTabs& t, const BlkLds& w, const SpState& s, int r) {
+  const BlkRxn h = t.rxn[r];
+  const double* ar = t.arr + 3 * r;
+  const double e = tab_exp(ar[0] + ar[1] * s.lnT - ar[2] * s.invT, t.exptab);
+  const double kf = (h.flags & 2) ? ar[0] : e;
+  double fwd = kf, pc = 1.0, dG = 0.0, dnu = 0.0;
+#pragma unroll
+  for (int q = 0; q < kBlkSide; ++q) {
+    fwd *= cpow_int(w.C[h.sp[q]], h.nu[q]);
+    dG -= h.nu[q] * w.g[h.sp[q]];
+    dnu -= h.nu[q];
+  }
+#pragma unroll
+  for (int q = kBlkSide; q < 2 * kBlkSide; ++q) {
+    pc *= cpow_int(w.C[h.sp[q]], h.nu[q]);
+    dG += h.nu[q] * w.g[h.sp[q]];
+    dnu += h.nu[q];
+  }
+  const double rev = (h.flags & 1) ? kf * tab_exp(dG - dnu * s.lnPfac, t.exptab) * pc : 0.0;
+  double pm = 1.0;
+  if (h.kind != kElementary) {
+    double M, a, b;
+    blk_pdep<false>(t, w, s, h, kf, 0.0, pm, M, a, b);
+  }
+  return (fwd - rev) * pm;
+}
+
+// dydt (constant pressure, dydt.c:7-133) on the workgroup; thread t returns component t.
+template <int NS>
+HPO_DI double blk_rhs(const BlkTabs& t, const BlkLds& w, double y, double P, int tid) {
+  const int K = t.K, R = t.R;
+  SpState s;
+  blk_prepare(t, w, y, P, tid, NS, s);
+  for (int r = tid; r < R; r += kBlkThreads) w.phi[r] = blk_rate(t, w, s, r);
+  __syncthreads();
+  double wdot = 0.0;
+  if (tid < K) {
+    const int b = t.csr_ptr[tid], e = t.csr_ptr[tid + 1];
+    for (int i = b; i < e; ++i) {
+      const uint32_t c = t.csr[i];
+      wdot += static_cast<double>(static_cast<int8_t>(c >> 16)) * w.phi[c & 0xFFFFu];
+    }
+  }
+  const bool sp = tid < K;
+  double v[2] = {sp ? s.hRT * kR * s.T * wdot : 0.0, sp ? s.Yk * s.cpR * kR * s.invW : 0.0};
+  bsum<2>(v, w.red, tid);   // Q, cpmix
+  // species row k lives in thread k + 1; the temperature row in thread 0
+  if (tid < K - 1) w.vec[tid + 1] = wdot * s.W * s.invrho;
+  if (tid == 0) w.vec[0] = -v[0] * s.invrho / v[1];
+  __syncthreads();
+  return tid < NS ? w.vec[tid] : 0.0;
+}
+
+// Analytic Jacobian (what jacob.c:3-371 computes): row t of J written to g.J[t * NS + j].
+template <int NS>
+HPO_DI void blk_jacobian(const BlkTabs& t, const BlkLds& w, const BlkScratch& g, double y, double P, int tid) {
+  const int K = t.K, R = t.R;
+  SpState s;
+  blk_prepare(t, w, y, P, tid, NS, s);
+  const double RT = kR * s.T;
+  // per reaction: d(phi)/dC slots, phi, d(phi)/dT, d(phi)/dm, reaction enthalpy
+  for (int r = tid; r < R; r += kBlkThreads) {
+    const BlkRxn h = t.rxn[r];
+    const double* ar = t.arr + 3 * r;
+    const bool constk = h.flags & 2;
+    const double kf = arrh(ar[0], ar[1], ar[2], constk, s.lnT, s.invT, t.exptab);
+    const double dlnkf = constk ? 0.0 : (ar[1] + ar[2] * s.invT) * s.invT;
+    double cr[kBlkSide], cq[kBlkSide];
+    double fwd = kf;
+#pragma unroll
+    for (int q = 0; q < kBlkSide; ++q) {
+      cr[q] = w.C[h.sp[q]];
+      fwd *= cpow_int(cr[q], h.nu[q]);
+    }
+    double kr = 0.0, rev = 0.0, dlnkr = 0.0, dG = 0.0, dnu = 0.0, dHRT = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlkSide; ++q) {
+      cq[q] = w.C[h.sp[kBlkSide + q]];
+      const int n = h.nu[kBlkSide + q], sq = h.sp[kBlkSide + q];
+      dG += n * w.g[sq];
+      dHRT += n * w.hRT[sq];
+      dnu += n;
+    }
+#pragma unroll
+    for (int q = 0; q < kBlkSide; ++q) {
+      const int n = h.nu[q], sq = h.sp[q];
+      dG -= n * w.g[sq];
+      dHRT -= n * w.hRT[sq];
+      dnu -= n;
+    }
+    if (h.flags & 1) {
+      kr = kf * tab_exp(dG - dnu * s.lnPfac, t.exptab);
+      rev = kr;
+#pragma unroll
+      for (int q = 0; q < kBlkSide; ++q) rev *= cpow_int(cq[q], h.nu[kBlkSide + q]);
+      dlnkr = dlnkf - (dHRT - dnu) * s.invT;
+    }
+    double pm = 1.0, M = 0.0, dpm_dM = 0.0, dpm_dT = 0.0, eff_base = 0.0;
+    int eb = 0, ee = 0;
+    if (h.kind != kElementary) {
+      blk_pdep<true>(t, w, s, h, kf, dlnkf, pm, M, dpm_dM, dpm_dT);
+      const PdepRec& pr = t.pd[h.pidx];
+      eff_base = pr.eff_base;
+      eb = pr.eff_begin;
+      ee = pr.eff_end;
+    }
+    const double dq = fwd - rev;
+    int sb = h.slot;
+#pragma unroll
+    for (int a = 0; a < kBlkSide; ++a)
+      if (h.nu[a]) {
+        double d = kf;
+#pragma unroll
+        for (int b = 0; b < kBlkSide; ++b)
+          if (h.nu[b]) d *= (b == a) ? dcpow_int(cr[b], h.nu[b]) : cpow_int(cr[b], h.nu[b]);
+        g.dphi[sb++] = pm * d;
+      }
+    if (h.flags & 1) {
+#pragma unroll
+      for (int a = 0; a < kBlkSide; ++a)
+        if (h.nu[kBlkSide + a]) {
+          double d = kr;
+#pragma unroll
+          for (int b = 0; b < kBlkSide; ++b)
+            if (h.nu[kBlkSide + b])
+              d *= (b == a) ? dcpow_int(cq[b], h.nu[kBlkSide + b]) : cpow_int(cq[b], h.nu[kBlkSide + b]);
+          g.dphi[sb++] = -pm * d;
+        }
+    }
+    const double fM = dq * dpm_dM;
+    for (int e = eb; e < ee; ++e) g.dphi[sb++] = fM * t.effv[e];
+    g.gphi[r] = dq * pm;
+    g.gdm[r] = fM * eff_base;
+    g.gdT[r] = pm * (fwd * dlnkf - rev * dlnkr) + dq * dpm_dT;
+    g.dH[r] = dHRT * RT;
+  }
+  __syncthreads();
+  // species rows (thread k): d(wdot_k)/dC row, wdot_k, d(wdot_k)/dT at fixed C and m,
+  // d(wdot_k)/dm; and the enthalpy-weighted column sum of column k
+  double* wdotv = w.sp4;
+  double* JTv = w.sp4 + ((K + 1) & ~1);
+  double* Jmv = w.sp4 + 2 * ((K + 1) & ~1);
+  double* sigv = w.sp4 + 3 * ((K + 1) & ~1);
+  double* dQv = w.sp4 + 4 * ((K + 1) & ~1);
+  double wdot = 0.0, JT = 0.0, Jm = 0.0, sigma = 0.0, dQdC = 0.0;
+  if (tid < K) {
+    double* row = g.JC + static_cast<int64_t>(tid) * K;
+    for (int i = 0; i < K; ++i) row[i] = 0.0;
+    const int b = t.csr_ptr[tid], e = t.csr_ptr[tid + 1];
+    for (int i = b; i < e; ++i) {
+      const uint32_t c = t.csr[i];
+      const int r = static_cast<int>(c & 0xFFFFu);
+      const double nu = static_cast<double>(static_cast<int8_t>(c >> 16));
+      wdot += nu * g.gphi[r];
+      JT += nu * g.gdT[r];
+      Jm += nu * g.gdm[r];
+      const BlkRxn& h = t.rxn[r];
+      for (int q = h.slot; q < h.slot + h.nslot; ++q) row[t.slot_sp[q]] += nu * g.dphi[q];
+    }
+    for (int i = 0; i < K; ++i) sigma += row[i] * w.C[i];
+    const uint32_t cb = t.col_ptr[tid], ce = t.col_ptr[tid + 1];
+    for (uint32_t i = cb; i < ce; ++i) {
+      const uint32_t c = t.col[i];
+      dQdC += g.dH[c >> 16] * g.dphi[c & 0xFFFFu];
+    }
+    wdotv[tid] = wdot;
+    JTv[tid] = JT;
+    Jmv[tid] = Jm;
+    sigv[tid] = sigma;
+    dQv[tid] = dQdC;
+  }
+  const bool sp = tid < K;
+  const double Hk = s.hRT * RT;
+  double v[7] = {sp ? Hk * wdot : 0.0, sp ? s.cpR * kR * wdot : 0.0, sp ? Hk * JT : 0.0, sp ? Hk * Jm : 0.0,
+                 sp ? s.Yk * s.cpR * kR * s.invW : 0.0, sp ? s.Yk * s.dcpR * kR * s.invW : 0.0,
+                 sp ? dQdC * w.C[tid] : 0.0};
+  bsum<7>(v, w.red, tid);   // also makes the species values above visible
+  const double Q = v[0], SCpw = v[1], SHJT = v[2], SHJm = v[3], cpmix = v[4], dcpmix = v[5], sigT = v[6];
+  // rows: thread 0 the temperature row, thread k + 1 the row of species k (chain rule to T, Y)
+  if (tid < NS) {
+    const double rho = s.rho, Wbar = s.Wbar;
+    const double invWN = t.sp[(K - 1) * kSpStride];
+    const double D = rho * cpmix, invD = 1.0 / D, fT = -Q * invD;
+    const double cpN = w.cp[K - 1];
+    double* Jrow = g.J + static_cast<int64_t>(tid) * NS;
+    if (tid == 0) {
+      const double dQdT = SCpw + SHJT - sigT * s.invT - (s.m * s.invT) * SHJm;
+      const double dDdT = -rho * cpmix * s.invT + rho * dcpmix;
+      Jrow[0] = (-dQdT - fT * dDdT) * invD;
+      const double c1 = dQv[K - 1] * rho * invWN;
+      for (int j = 0; j < NS - 1; ++j) {
+        const double invWj = t.sp[j * kSpStride];
+        const double dj = invWj - invWN;
+        const double base = dQv[j] * rho * invWj - c1 - Wbar * dj * sigT;
+        const double dDdY = -rho * Wbar * dj * cpmix + rho * (w.cp[j] - cpN);
+        Jrow[j + 1] = (-base - fT * dDdY) * invD;
+      }
+    } else {
+      const int k = tid - 1;
+      const double* row = g.JC + static_cast<int64_t>(k) * K;
+      const double wk = wdotv[k], sk = sigv[k];
+      const double fac = t.sp[k * kSpStride + 1] / rho;
+      Jrow[0] = fac * (JTv[k] - sk * s.invT - (s.m * s.invT) * Jmv[k] + wk * s.invT);
+      const double c1 = row[K - 1] * rho * invWN;
+      for (int j = 0; j < NS - 1; ++j) {
+        const double invWj = t.sp[j * kSpStride];
+        const double dj = invWj - invWN;
+        const double base = row[j] * rho * invWj - c1 - Wbar * dj * sk;
+        Jrow[j + 1] = fac * (base + wk * Wbar * dj);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Newton matrix: Gauss-Jordan inverse across the workgroup, one register row per thread
+// ---------------------------------------------------------------------------
+template <int NS, typename F>
+struct BlkInv {
+  F a[NS];     // this thread's row; after the inversion, row `step` of (P M)^-1
+  int step;    // the elimination step at which this row was the pivot (-1: none)
+};
+
+HPO_DI unsigned blk_pivot_key(double v, int tid) {
+  const float f = fabsf(static_cast<float>(v));
+  return (static_cast<unsigned>(__float_as_int(f)) & ~255u) | static_cast<unsigned>(255 - tid);
+}
+HPO_DI unsigned blk_pivot_key(float v, int tid) {
+  return (static_cast<unsigned>(__float_as_int(fabsf(v))) & ~255u) | static_cast<unsigned>(255 - tid);
+}
+
+// Pivot: the largest |a[k]| among rows not yet pivoted (compared at single precision, the
+// lower thread on ties). Every row, earlier pivot rows included, then subtracts l times the
+// pivot row; column k takes the inverse's entries. Returns false for a singular matrix.
+template <int NS, typename F>
+HPO_DI bool blk_gj_invert(BlkInv<NS, F>& L, const BlkLds& w, int tid) {
+  bool done = tid >= NS;
+  bool singular = false;
+  F rd = F(0);
+  L.step = -1;
+  F* prow = reinterpret_cast<F*>(w.prow);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const unsigned key = done ? 0u : blk_pivot_key(L.a[k], tid);
+    const unsigned kmax = bmax_u(key, w.iscal, tid);
+    const int p = 255 - static_cast<int>(kmax & 255u);
+    const bool isp = tid == p;
+    if (isp) {
+#pragma unroll
+      for (int j = 0; j < NS; ++j) prow[j] = L.a[j];
+      if (tid == p) w.perm[k] = p;
+    }
+    __syncthreads();
+    const F piv = prow[k];
+    const bool bad = (kmax == 0u) || !(fabs(static_cast<double>(piv)) > 0.0);
+    singular = singular || bad;
+    F mult = bad ? F(0) : F(1) / piv;
+    if (isp && !bad) {
+      done = true;
+      rd = mult;
+      L.step = k;
+    }
+    const F l = isp ? F(0) : L.a[k] * mult;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      if (j == k) continue;
+      L.a[j] = fma(-l, prow[j], L.a[j]);
+    }
+    L.a[k] = isp ? F(1) : -l;
+  }
+#pragma unroll
+  for (int j = 0; j < NS; ++j) L.a[j] = (tid < NS) ? L.a[j] * rd : F(0);
+  __syncthreads();
+  return !singular;
+}
+
+// x = M^-1 b (b, x: component t in thread t). Thread p_k holds row k of (P M)^-1, whose
+// columns multiply b permuted to pivot order: b_{p_j} sits at position j, which thread p_j
+// writes directly (its step is j). The product is x_k, sent to thread k.
+template <int NS, typename F>
+HPO_DI double blk_gj_apply(const BlkInv<NS, F>& L, double b, const BlkLds& w, int tid) {
+  __syncthreads();   // previous readers of vec / vec2 are done
+  if (L.step >= 0) w.vec2[L.step] = b;
+  __syncthreads();
+  F x0 = F(0), x1 = F(0);
+#pragma unroll
+  for (int j = 0; j + 1 < NS; j += 2) {
+    x0 = fma(L.a[j], static_cast<F>(w.vec2[j]), x0);
+    x1 = fma(L.a[j + 1], static_cast<F>(w.vec2[j + 1]), x1);
+  }
+  if (NS & 1) x0 = fma(L.a[NS - 1], static_cast<F>(w.vec2[NS - 1]), x0);
+  if (L.step >= 0) w.vec[L.step] = static_cast<double>(x0 + x1);
+  __syncthreads();
+  return tid < NS ? w.vec[tid] : 0.0;
+}
+
+template <int NS, bool LU32>
+struct BlockLayout {
+  using F = typename std::conditional<LU32, float, double>::type;
+  const BlkTabs& t;
+  const BlkLds& w;
+  const BlkScratch& g;
+  const int tid;
+  const int wave_lane;
+  const double* exptab;
+  BlkInv<NS, F> inv;
+
+  HPO_DI BlockLayout(const BlkTabs& t_, const BlkLds& w_, const BlkScratch& g_, int tid_)
+      : t(t_), w(w_), g(g_), tid(tid_), wave_lane(tid_ & 63), exptab(t_.exptab) {}
+  HPO_DI bool any(bool c) const { return bsum1(c ? 1.0 : 0.0, w.red, tid) > 0.0; }
+  HPO_DI double sum(double v) const { return bsum1(v, w.red, tid); }
+  HPO_DI double max(double v) const { return bmax(v, w.red, tid); }
+  HPO_DI double rhs(double y, double P) const { return blk_rhs<NS>(t, w, y, P, tid); }
+
+  template <bool PROF>
+  HPO_DI bool setup(double y, double P, double gamma, bool jnew, long long& cj) {
+    if (jnew) {
+      const long long c0 = PROF ? clock64() : 0;
+      blk_jacobian<NS>(t, w, g, y, P, tid);
+      if (PROF) cj = clock64() - c0;
+    }
+    const double* Jrow = g.J + static_cast<int64_t>(tid < NS ? tid : 0) * NS;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const double m = -gamma * Jrow[j] + ((j == tid) ? 1.0 : 0.0);
+      inv.a[j] = tid < NS ? static_cast<F>(m) : F(0);
+    }
+    return blk_gj_invert<NS, F>(inv, w, tid);
+  }
+
+  // x = M^-1 b; with the FP32 inverse, `sweeps` FP64 residual sweeps r = b - (I - gamma J) x
+  // against the FP64 Jacobian rows in scratch.
+  HPO_DI double solve(double b, double gl, int sweeps) {
+    double x = blk_gj_apply<NS, F>(inv, b, w, tid);
+    if constexpr (LU32) {
+      const double* Jrow = g.J + static_cast<int64_t>(tid < NS ? tid : 0) * NS;
+      for (int sw = 0; sw < sweeps; ++sw) {
+        __syncthreads();
+        w.y[tid] = x;
+        __syncthreads();
+        double Jx = 0.0;
+        for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j], w.y[j], Jx);
+        const double r = (tid < NS) ? b - x + gl * Jx : 0.0;
+        x += blk_gj_apply<NS, F>(inv, r, w, tid);
+      }
+    }
+    return x;
+  }
+};
+
+template <int NS, bool LU32>
+constexpr int blk_waves_per_eu() {
+  return LU32 ? HPO_BLK_WAVES_PER_EU_FP32 : HPO_BLK_WAVES_PER_EU_FP64;
+}
+
+template <int NS, bool LU32, bool PROF>
+__global__ __launch_bounds__(kBlkThreads) __attribute__((
+    amdgpu_waves_per_eu(blk_waves_per_eu<NS, LU32>(), blk_waves_per_eu<NS, LU32>()))) void bdf_block_kernel(BdfArgs A) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const BlkLayout& Lay = A.blay;
+  for (int i = threadIdx.x * 16; i < Lay.lds_bytes; i += blockDim.x * 16)
+    *reinterpret_cast<uint4*>(smem + i) = *reinterpret_cast<const uint4*>(A.mech + i);
+  __syncthreads();
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  BlkTabs t = make_blk_tabs(smem, A.mech, Lay);
+  double* wbase = reinterpret_cast<double*>(smem + Lay.lds_bytes);
+  BlkLds w = make_blk_lds(wbase, Lay.K, Lay.R);
+  Scal& S = *reinterpret_cast<Scal*>(wbase + blk_lds_doubles(Lay.K, Lay.R) + wave * kScalDoubles);
+  BlkScratch g = make_blk_scratch(A.scratch + static_cast<int64_t>(blockIdx.x) * A.scratch_stride, Lay);
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) w.iscal[8] = static_cast<int>(atomicAdd(A.queue, 1u));
+    __syncthreads();
+    const unsigned c = static_cast<unsigned>(w.iscal[8]);
+    if (static_cast<int64_t>(c) >= A.ncells) break;
+    const int64_t cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
+    BlockLayout<NS, LU32> Ly(t, w, g, tid);
+    integrate_cell<NS, LU32, PROF>(A, Ly, S, cell);
+  }
+}
+
+// ---- component kernels used by the parity tests ----
+template <int NS, int MODE>
+__global__ __launch_bounds__(kBlkThreads) void blk_eval_kernel(EvalArgs A) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const BlkLayout& Lay = A.blay;
+  for (int i = threadIdx.x * 16; i < Lay.lds_bytes; i += blockDim.x * 16)
+    *reinterpret_cast<uint4*>(smem + i) = *reinterpret_cast<const uint4*>(A.mech + i);
+  __syncthreads();
+  const int tid = threadIdx.x;
+  BlkTabs t = make_blk_tabs(smem, A.mech, Lay);
+  BlkLds w = make_blk_lds(reinterpret_cast<double*>(smem + Lay.lds_bytes), Lay.K, Lay.R);
+  BlkScratch g = make_blk_scratch(A.scratch + static_cast<int64_t>(blockIdx.x) * A.scratch_stride, Lay);
+  for (int64_t c = blockIdx.x; c < A.ncells; c += gridDim.x) {
+    const double y = tid < NS ? A.y[c * A.ld + tid] : 0.0;
+    const double P = A.P[c];
+    if constexpr (MODE == 0) {
+      const double f = blk_rhs<NS>(t, w, y, P, tid);
+      if (tid < NS) A.out[c * A.ld + tid] = f;
+    } else {
+      blk_jacobian<NS>(t, w, g, y, P, tid);
+      if (tid < NS)
+        for (int j = 0; j < NS; ++j) A.out[(c * NS + tid) * NS + j] = g.J[tid * NS + j];
+    }
+  }
+}
+
+template <int NS, typename F>
+__global__ __launch_bounds__(kBlkThreads) void blk_gj_kernel(LuArgs A) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const int tid = threadIdx.x;
+  BlkLds w = make_blk_lds(reinterpret_cast<double*>(smem), NS, 1);
+  const int64_t c = blockIdx.x;
+  BlkInv<NS, F> inv;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) inv.a[j] = tid < NS ? static_cast<F>(A.M[(c * NS + tid) * NS + j]) : F(0);
+  const bool ok = blk_gj_invert<NS, F>(inv, w, tid);
+  const double b = tid < NS ? A.b[c * NS + tid] : 0.0;
+  const double x = blk_gj_apply<NS, F>(inv, b, w, tid);
+  if (tid < NS) A.x[c * NS + tid] = x;
+  if (tid == 0) A.ok[c] = ok ? 1 : 0;
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host)
+// ---------------------------------------------------------------------------
+#define HPO_FOR_EACH_BLK_NS(X) X(53) X(100)
+
+bool block_kernel_supports(int ns) {
+#define HPO_CASE(N) \
+  if (ns == N) return true;
+  HPO_FOR_EACH_BLK_NS(HPO_CASE)
+#undef HPO_CASE
+  return false;
+}
+
+int blk_work_lds_bytes(const BlkLayout& L) {
+  return (dev::blk_lds_doubles(L.K, L.R) + dev::kBlkWaves * dev::kScalDoubles) * 8;
+}
+int64_t blk_scratch_doubles_host(const BlkLayout& L) { return dev::blk_scratch_doubles(L, L.K); }
+
+template <int NS>
+static const void* blk_fn(bool lu32, bool prof) {
+  if (lu32) return prof ? reinterpret_cast<const void*>(&dev::bdf_block_kernel<NS, true, true>)
+                        : reinterpret_cast<const void*>(&dev::bdf_block_kernel<NS, true, false>);
+  return prof ? reinterpret_cast<const void*>(&dev::bdf_block_kernel<NS, false, true>)
+              : reinterpret_cast<const void*>(&dev::bdf_block_kernel<NS, false, false>);
+}
+
+const void* bdf_block_kernel_fn(int ns, int lu32, int prof) {
+#define HPO_CASE(N) \
+  if (ns == N) return blk_fn<N>(lu32 != 0, prof != 0);
+  HPO_FOR_EACH_BLK_NS(HPO_CASE)
+#undef HPO_CASE
+  return nullptr;
+}
+
+hipError_t launch_bdf_block(int ns, const BdfArgs& A, int lu32, int grid, size_t shmem, hipStream_t st) {
+  const void* fn = bdf_block_kernel_fn(ns, lu32, A.prof != nullptr);
+  if (!fn) return hipErrorInvalidValue;
+  void* args[] = {const_cast<BdfArgs*>(&A)};
+  return hipLaunchKernel(fn, dim3(grid), dim3(dev::kBlkThreads), args, shmem, st);
+}
+
+hipError_t launch_eval_block(int ns, const EvalArgs& A, int grid, size_t shmem, hipStream_t st) {
+#define HPO_CASE(N)                                                                                       \
+  if (ns == N) {                                                                                          \
+    if (A.mode == 0)                                                                                      \
+      hipLaunchKernelGGL((dev::blk_eval_kernel<N, 0>), dim3(grid), dim3(dev::kBlkThreads), shmem, st, A); \
+    else                                                                                                  \
+      hipLaunchKernelGGL((dev::blk_eval_kernel<N, 1>), dim3(grid), dim3(dev::kBlkThreads), shmem, st, A); \
+    return hipGetLastError();                                                                             \
+  }
+  HPO_FOR_EACH_BLK_NS(HPO_CASE)
+#undef HPO_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_gj_block(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st) {
+  if (ncases <= 0) return hipSuccess;
+  const size_t shmem = static_cast<size_t>(dev::blk_lds_doubles(ns, 1)) * 8;
+#define HPO_CASE(N)                                                                                              \
+  if (ns == N) {                                                                                                 \
+    if (fp32)                                                                                                    \
+      hipLaunchKernelGGL((dev::blk_gj_kernel<N, float>), dim3(ncases), dim3(dev::kBlkThreads), shmem, st, A);   \
+    else                                                                                                         \
+      hipLaunchKernelGGL((dev::blk_gj_kernel<N, double>), dim3(ncases), dim3(dev::kBlkThreads), shmem, st, A);  \
+    return hipGetLastError();                                                                                    \
+  }
+  HPO_FOR_EACH_BLK_NS(HPO_CASE)
+#undef HPO_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace hpo

@@ -48,6 +48,7 @@ hpo_mech::~hpo_mech() {
   if (device >= 0) {
     (void)hipSetDevice(device);
     if (d_blob) (void)hipFree(d_blob);
+    if (d_bblob) (void)hipFree(d_bblob);
     if (d_scratch) (void)hipFree(d_scratch);
     if (d_queue) (void)hipFree(d_queue);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -57,29 +58,37 @@ hpo_mech::~hpo_mech() {
 }
 
 void hpo_mech::ensure_device() {
-  if (d_blob) return;
+  if (d_queue) return;
   HPO_HIP(hipGetDevice(&device));
   hipDeviceProp_t prop;
   HPO_HIP(hipGetDeviceProperties(&prop, device));
   n_cu = prop.multiProcessorCount;
-  d_blob = dalloc<uint8_t>(blob.bytes.size());
-  HPO_HIP(hipMemcpy(d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice));
+  if (has_wave) {
+    d_blob = dalloc<uint8_t>(blob.bytes.size());
+    HPO_HIP(hipMemcpy(d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice));
+    scratch_stride = hpo::wave_scratch_doubles_host(blob.lay);
+  }
+  if (has_block) {
+    d_bblob = dalloc<uint8_t>(bblob.bytes.size());
+    HPO_HIP(hipMemcpy(d_bblob, bblob.bytes.data(), bblob.bytes.size(), hipMemcpyHostToDevice));
+  }
   d_queue = dalloc<uint32_t>(4);
   HPO_HIP(hipEventCreate(&ev0));
   HPO_HIP(hipEventCreate(&ev1));
   HPO_HIP(hipEventCreateWithFlags(&ev_last, hipEventDisableTiming));
-  scratch_stride = hpo::wave_scratch_doubles_host(blob.lay);
 }
 
-void hpo_mech::ensure_scratch(int64_t slots) {
-  if (slots <= scratch_slots) return;
+void hpo_mech::ensure_scratch_doubles(int64_t n) {
+  if (n <= scratch_cap) return;
   // the previous launch on this scratch must be done before it is freed
   if (launched) HPO_HIP(hipEventSynchronize(ev_last));
   if (d_scratch) HPO_HIP(hipFree(d_scratch));
   d_scratch = nullptr;
-  d_scratch = dalloc<double>(static_cast<size_t>(slots * scratch_stride));
-  scratch_slots = slots;
+  d_scratch = dalloc<double>(static_cast<size_t>(n));
+  scratch_cap = n;
 }
+
+void hpo_mech::ensure_scratch(int64_t slots) { ensure_scratch_doubles(slots * scratch_stride); }
 
 struct hpo_batch {
   const hpo_mech* mech = nullptr;
@@ -110,6 +119,24 @@ void check_opts(const hpo_opts& o) {
   if (o.waves_per_block < 0 || o.waves_per_block > 4) throw std::invalid_argument("waves_per_block must be in 0..4");
   if (o.blocks_per_cu < 0) throw std::invalid_argument("blocks_per_cu must be >= 0");
   if (o.lu_layout != 0 && o.lu_layout != 1) throw std::invalid_argument("lu_layout must be 0 or 1");
+  if (o.cell_layout < 0 || o.cell_layout > 2) throw std::invalid_argument("cell_layout must be 0, 1 or 2");
+}
+
+bool use_block_layout(const hpo_mech* m, const hpo_opts& o) {
+  const int ns = m->host.K();
+  const bool wave_ok = m->has_wave && hpo::kernel_supports(ns);
+  const bool block_ok = m->has_block && hpo::block_kernel_supports(ns);
+  if (o.cell_layout == 1) {
+    if (!wave_ok) throw std::invalid_argument("no wave-per-cell kernel for a state of " + std::to_string(ns) + " variables");
+    return false;
+  }
+  if (o.cell_layout == 2) {
+    if (!block_ok) throw std::invalid_argument("no block-per-cell kernel for a state of " + std::to_string(ns) + " variables");
+    return true;
+  }
+  if (wave_ok) return false;
+  if (block_ok) return true;
+  throw std::invalid_argument("no kernel instantiated for a state of " + std::to_string(ns) + " variables");
 }
 
 // Launches the persistent BDF kernel on arrays already resident on the device.
@@ -118,18 +145,26 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
                       const int32_t* d_order, int64_t* d_prof, const hpo_opts& o, hipStream_t st,
                       hpo_timing* tm, const ExecRes* res) {
   check_opts(o);
-  const int ns = m->blob.lay.K;
-  if (!hpo::kernel_supports(ns))
-    throw std::invalid_argument("no kernel instantiated for a state of " + std::to_string(ns) + " variables");
+  const int ns = m->host.K();
+  const bool block_layout = use_block_layout(m, o);
   if (ncells < 0 || ncells > 0x7fffffffLL) throw std::invalid_argument("ncells out of range");
   if (ld < ns) throw std::invalid_argument("leading dimension smaller than the state");
   m->ensure_device();
-  const int wpb = o.waves_per_block > 0 ? o.waves_per_block : 4;
-  const int block = 64 * wpb;
-  const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) +
-                     static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
+  int wpb, block;
+  size_t lds;
+  const void* fn;
   const int variant = hpo::bdf_variant(o.lu_precision, o.lu_layout);
-  const void* fn = hpo::bdf_kernel_fn(ns, variant, d_prof != nullptr);
+  if (block_layout) {
+    wpb = 1;   // one cell per 256-thread workgroup
+    block = 256;
+    lds = static_cast<size_t>(m->bblob.lay.lds_bytes) + hpo::blk_work_lds_bytes(m->bblob.lay);
+    fn = hpo::bdf_block_kernel_fn(ns, o.lu_precision, d_prof != nullptr);
+  } else {
+    wpb = o.waves_per_block > 0 ? o.waves_per_block : 4;
+    block = 64 * wpb;
+    lds = static_cast<size_t>(m->blob.lay.lds_bytes) + static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
+    fn = hpo::bdf_kernel_fn(ns, variant, d_prof != nullptr);
+  }
   int occ = 0;
   HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
   if (occ < 1) throw std::invalid_argument("kernel does not fit on a CU with this configuration");
@@ -138,6 +173,9 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   if (res && res->share_pct < 100) grid = (grid * std::max(res->share_pct, 1) + 99) / 100;
   grid = std::min<int64_t>(grid, (ncells + wpb - 1) / wpb);
   grid = std::max<int64_t>(grid, 1);
+  // scratch: per wave slot (wave layout) or per workgroup (block layout)
+  const int64_t stride = block_layout ? hpo::blk_scratch_doubles_host(m->bblob.lay) : m->scratch_stride;
+  const int64_t need = grid * wpb * stride;
   ExecRes own;
   std::unique_lock<std::mutex> own_lock;
   bool own_ctx = false, capturing = false;
@@ -148,22 +186,21 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HPO_HIP(hipStreamIsCapturing(st, &cs));
     capturing = cs != hipStreamCaptureStatusNone;
-    m->ensure_scratch(grid * wpb);
+    m->ensure_scratch_doubles(need);
     own.d_queue = m->d_queue;
     own.d_scratch = m->d_scratch;
-    own.scratch_slots = m->scratch_slots;
-    own.scratch_stride = m->scratch_stride;
+    own.scratch_doubles = m->scratch_cap;
     res = &own;
     // a captured launch is not ordered against eager ones here: its replays must not overlap
     // other launches on this handle (the header says so)
     if (!capturing && m->launched && m->last_stream != st) HPO_HIP(hipStreamWaitEvent(st, m->ev_last, 0));
   }
-  if (res->scratch_slots < grid * wpb || res->scratch_stride < m->scratch_stride)
-    throw std::invalid_argument("launch context scratch too small for this launch");
+  if (res->scratch_doubles < need) throw std::invalid_argument("launch context scratch too small for this launch");
 
   hpo::BdfArgs A{};
-  A.mech = m->d_blob;
+  A.mech = block_layout ? m->d_bblob : m->d_blob;
   A.lay = m->blob.lay;
+  A.blay = m->bblob.lay;
   A.y_in = d_yin;
   A.y_out = d_yout;
   A.P = d_P;
@@ -173,7 +210,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   A.order = d_order;
   A.queue = res->d_queue;
   A.scratch = res->d_scratch;
-  A.scratch_stride = res->scratch_stride;
+  A.scratch_stride = stride;
   A.ncells = ncells;
   A.ld = ld;
   A.rtol = o.rtol;
@@ -184,7 +221,12 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
 
   HPO_HIP(hpo::launch_reset_counter(res->d_queue, st));
   if (tm) HPO_HIP(hipEventRecord(m->ev0, st));
-  if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
+  if (ncells > 0) {
+    if (block_layout)
+      HPO_HIP(hpo::launch_bdf_block(ns, A, o.lu_precision, static_cast<int>(grid), lds, st));
+    else
+      HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
+  }
   if (tm) HPO_HIP(hipEventRecord(m->ev1, st));
   if (own_ctx && !capturing) {
     HPO_HIP(hipEventRecord(m->ev_last, st));
@@ -203,21 +245,31 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   }
 }
 
-int64_t max_wave_slots(hpo_mech* m) {
+int64_t max_scratch_doubles(hpo_mech* m) {
   m->ensure_device();
-  const int ns = m->blob.lay.K;
-  int64_t most = 1;
-  // one-wave workgroups pack the most waves onto a CU; the launch grid never exceeds
-  // n_CU x (workgroups per CU) x (waves per workgroup)
-  for (int wpb = 1; wpb <= 4; ++wpb)
-    for (int variant = 0; variant < 3; ++variant) {
-      const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) +
-                         static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
+  const int ns = m->host.K();
+  int64_t most = 0;
+  if (m->has_wave && hpo::kernel_supports(ns)) {
+    // one-wave workgroups pack the most waves onto a CU; the launch grid never exceeds
+    // n_CU x (workgroups per CU) x (waves per workgroup)
+    for (int wpb = 1; wpb <= 4; ++wpb)
+      for (int variant = 0; variant < 3; ++variant) {
+        const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) +
+                           static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
+        int occ = 0;
+        HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_kernel_fn(ns, variant), 64 * wpb, lds));
+        most = std::max<int64_t>(most, static_cast<int64_t>(m->n_cu) * occ * wpb * m->scratch_stride);
+      }
+  }
+  if (m->has_block && hpo::block_kernel_supports(ns)) {
+    const size_t lds = static_cast<size_t>(m->bblob.lay.lds_bytes) + hpo::blk_work_lds_bytes(m->bblob.lay);
+    for (int lu32 = 0; lu32 < 2; ++lu32) {
       int occ = 0;
-      HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_kernel_fn(ns, variant), 64 * wpb, lds));
-      most = std::max<int64_t>(most, static_cast<int64_t>(occ) * wpb);
+      HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_block_kernel_fn(ns, lu32, 0), 256, lds));
+      most = std::max<int64_t>(most, static_cast<int64_t>(m->n_cu) * occ * hpo::blk_scratch_doubles_host(m->bblob.lay));
     }
-  return static_cast<int64_t>(m->n_cu) * most;
+  }
+  return std::max<int64_t>(most, 1);
 }
 
 }  // namespace capi
@@ -243,7 +295,22 @@ static int mech_load(const char* path, hpo_mech** out, bool by_extension) {
   try {
     auto m = std::make_unique<hpo_mech>();
     m->host = by_extension ? hpo::load_mechanism(path) : hpo::load_mechanism_yaml(path);
-    m->blob = hpo::build_device_tables(m->host);
+    // wave-per-cell tables (at most 63 species) and block-per-cell tables (at most 255); the
+    // mechanism loads when either fits, and the launch picks the layout (hpo_opts::cell_layout)
+    std::string wave_err, block_err;
+    try {
+      m->blob = hpo::build_device_tables(m->host);
+      m->has_wave = true;
+    } catch (const std::exception& e) {
+      wave_err = e.what();
+    }
+    try {
+      m->bblob = hpo::build_block_tables(m->host);
+      m->has_block = true;
+    } catch (const std::exception& e) {
+      block_err = e.what();
+    }
+    if (!m->has_wave && !m->has_block) throw std::runtime_error(wave_err + "; " + block_err);
     *out = m.release();
     return HPO_OK;
   } catch (...) {
@@ -441,9 +508,9 @@ int hpo_integrate_device_noalloc(const hpo_mech* mc, int64_t ncells, const doubl
   try {
     check_opts(opt);
     const hpo_mech* m = mc;
-    const int ns = m->blob.lay.K;
-    if (!hpo::kernel_supports(ns))
-      throw std::invalid_argument("no kernel instantiated for a state of " + std::to_string(ns) + " variables");
+    const int ns = m->host.K();
+    if (use_block_layout(m, opt))
+      throw std::invalid_argument("the no-allocation entry point runs the wave-per-cell kernel only");
     if (ncells < 0 || ncells > 0x7fffffffLL) throw std::invalid_argument("ncells out of range");
     if (ld < ns) throw std::invalid_argument("leading dimension smaller than the state");
     int device = -1;
@@ -500,12 +567,19 @@ int hpo_integrate_device_noalloc(const hpo_mech* mc, int64_t ncells, const doubl
   }
 }
 
-static int eval_common(const hpo_mech* mc, int64_t n, const double* y, const double* P, double* out, int mode) {
+// cell_layout: 0 = the wave tables when the mechanism has them, else the block tables;
+// 1 = wave (one wave per cell); 2 = block (one 256-thread workgroup per cell)
+static int eval_common(const hpo_mech* mc, int32_t cell_layout, int64_t n, const double* y, const double* P,
+                       double* out, int mode) {
   if (!mc || !y || !P || !out || n < 0) return fail(HPO_E_INVALID, "bad argument");
-  if (n == 0) return HPO_OK;
+  if (cell_layout < 0 || cell_layout > 2) return fail(HPO_E_INVALID, "cell_layout must be 0, 1 or 2");
   hpo_mech* m = const_cast<hpo_mech*>(mc);
-  const int ns = m->blob.lay.K;
-  if (!hpo::kernel_supports(ns)) return fail(HPO_E_UNSUPPORTED, "no kernel for this state size");
+  const int ns = m->host.K();
+  const bool wave_ok = m->has_wave && hpo::kernel_supports(ns);
+  const bool block_ok = m->has_block && hpo::block_kernel_supports(ns);
+  const bool block = cell_layout == 2 || (cell_layout == 0 && !wave_ok);
+  if (block ? !block_ok : !wave_ok) return fail(HPO_E_UNSUPPORTED, "no kernel of this layout for this state size");
+  if (n == 0) return HPO_OK;
   double *dy = nullptr, *dP = nullptr, *dout = nullptr;
   int rc = HPO_OK;
   try {
@@ -513,9 +587,10 @@ static int eval_common(const hpo_mech* mc, int64_t n, const double* y, const dou
     // the handle's scratch: after any integration still running on it
     std::lock_guard<std::mutex> g(m->launch_mu);
     if (m->launched) HPO_HIP(hipEventSynchronize(m->ev_last));
-    const int wpb = 4, block = 256;
-    int64_t grid = std::min<int64_t>((n + wpb - 1) / wpb, 4096);
-    m->ensure_scratch(grid * wpb);
+    const int wpb = block ? 1 : 4;
+    const int64_t grid = std::min<int64_t>((n + wpb - 1) / wpb, block ? 1024 : 4096);
+    const int64_t stride = block ? hpo::blk_scratch_doubles_host(m->bblob.lay) : m->scratch_stride;
+    m->ensure_scratch_doubles(grid * wpb * stride);
     size_t outn = mode == 0 ? static_cast<size_t>(n * ns) : static_cast<size_t>(n * ns * ns);
     dy = dalloc<double>(static_cast<size_t>(n * ns));
     dP = dalloc<double>(n);
@@ -523,18 +598,24 @@ static int eval_common(const hpo_mech* mc, int64_t n, const double* y, const dou
     HPO_HIP(hipMemcpy(dy, y, n * ns * sizeof(double), hipMemcpyHostToDevice));
     HPO_HIP(hipMemcpy(dP, P, n * sizeof(double), hipMemcpyHostToDevice));
     hpo::EvalArgs A{};
-    A.mech = m->d_blob;
+    A.mech = block ? m->d_bblob : m->d_blob;
     A.lay = m->blob.lay;
+    A.blay = m->bblob.lay;
     A.y = dy;
     A.P = dP;
     A.out = dout;
     A.scratch = m->d_scratch;
-    A.scratch_stride = m->scratch_stride;
+    A.scratch_stride = stride;
     A.ncells = n;
     A.ld = ns;
     A.mode = mode;
-    size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) + wpb * hpo::eval_wave_lds_doubles(m->blob.lay) * 8;
-    HPO_HIP(hpo::launch_eval(ns, A, static_cast<int>(grid), block, lds, nullptr));
+    if (block) {
+      const size_t lds = static_cast<size_t>(m->bblob.lay.lds_bytes) + hpo::blk_work_lds_bytes(m->bblob.lay);
+      HPO_HIP(hpo::launch_eval_block(ns, A, static_cast<int>(grid), lds, nullptr));
+    } else {
+      const size_t lds = static_cast<size_t>(m->blob.lay.lds_bytes) + wpb * hpo::eval_wave_lds_doubles(m->blob.lay) * 8;
+      HPO_HIP(hpo::launch_eval(ns, A, static_cast<int>(grid), 64 * wpb, lds, nullptr));
+    }
     HPO_HIP(hipDeviceSynchronize());
     HPO_HIP(hipMemcpy(out, dout, outn * sizeof(double), hipMemcpyDeviceToHost));
   } catch (...) {
@@ -547,11 +628,17 @@ static int eval_common(const hpo_mech* mc, int64_t n, const double* y, const dou
 }
 
 int hpo_eval_rhs(const hpo_mech* m, int64_t n, const double* y, const double* P, double* f) {
-  return eval_common(m, n, y, P, f, 0);
+  return eval_common(m, 0, n, y, P, f, 0);
 }
 
 int hpo_eval_jac(const hpo_mech* m, int64_t n, const double* y, const double* P, double* J) {
-  return eval_common(m, n, y, P, J, 1);
+  return eval_common(m, 0, n, y, P, J, 1);
+}
+
+int hpo_eval_layout(const hpo_mech* m, int32_t cell_layout, int32_t what, int64_t n, const double* y,
+                    const double* P, double* out) {
+  if (what != 0 && what != 1) return fail(HPO_E_INVALID, "what must be 0 (RHS) or 1 (Jacobian)");
+  return eval_common(m, cell_layout, n, y, P, out, what);
 }
 
 int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double* M, const double* b, double* x, int32_t* ok) {
@@ -584,9 +671,11 @@ int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double* M, const dou
 int hpo_lu_solve_layout(int32_t ns, int32_t fp32, int32_t layout, int64_t n, const double* M, const double* b,
                         double* x, int32_t* ok, float* kernel_ms) {
   if (!M || !b || !x || !ok || n < 0) return fail(HPO_E_INVALID, "bad argument");
-  if (layout != 0 && layout != 1) return fail(HPO_E_INVALID, "layout must be 0 (wave per matrix) or 1 (workgroup per matrix)");
-  if (layout == 1 && fp32) return fail(HPO_E_UNSUPPORTED, "the workgroup-per-matrix layout is FP64 only");
-  if (!hpo::kernel_supports(ns)) return fail(HPO_E_UNSUPPORTED, "no kernel for this size");
+  if (layout < 0 || layout > 2)
+    return fail(HPO_E_INVALID, "layout must be 0 (wave per matrix), 1 (workgroup LU) or 2 (workgroup Gauss-Jordan)");
+  if (layout == 1 && fp32) return fail(HPO_E_UNSUPPORTED, "the workgroup LU layout is FP64 only");
+  if (layout == 2 ? !hpo::block_kernel_supports(ns) : !hpo::kernel_supports(ns))
+    return fail(HPO_E_UNSUPPORTED, "no kernel for this size");
   if (n > 0x7fffffffLL) return fail(HPO_E_INVALID, "too many matrices for one launch");
   if (kernel_ms) *kernel_ms = 0.f;
   if (n == 0) return HPO_OK;
@@ -607,8 +696,10 @@ int hpo_lu_solve_layout(int32_t ns, int32_t fp32, int32_t layout, int64_t n, con
     HPO_HIP(hipEventRecord(e0, nullptr));
     if (layout == 0)
       HPO_HIP(hpo::launch_lu(ns, fp32, A, n, nullptr));
-    else
+    else if (layout == 1)
       HPO_HIP(hpo::launch_lu_block(ns, A, n, nullptr));
+    else
+      HPO_HIP(hpo::launch_gj_block(ns, fp32, A, n, nullptr));
     HPO_HIP(hipEventRecord(e1, nullptr));
     HPO_HIP(hipEventSynchronize(e1));
     if (kernel_ms) HPO_HIP(hipEventElapsedTime(kernel_ms, e0, e1));
@@ -683,7 +774,7 @@ int hpo_autotune(const hpo_mech* mc, int64_t ncells, const double* d_y, int64_t 
   try {
     check_opts(*o);
     m->ensure_device();
-    const int64_t ns = m->blob.lay.K;
+    const int64_t ns = m->host.K();
     if (ld < ns) throw std::invalid_argument("leading dimension smaller than the state");
     // evenly strided sample
     const int64_t ns_req = sample_cells > 0 ? sample_cells : 32768;
@@ -740,45 +831,58 @@ int hpo_autotune(const hpo_mech* mc, int64_t ncells, const double* d_y, int64_t 
 
     hpo_tune_report R{};
     R.sample_cells = S;
+    const bool block_only = use_block_layout(m, base);
+    const bool block_too = !block_only && o->cell_layout == 0 && m->has_block && hpo::block_kernel_supports(ns);
     const int wpbs[3] = {4, 2, 1};
     double best_ms = 1e300;
     int best = -1;
-    for (int wi = 0; wi < 3; ++wi)
+    auto record = [&](const hpo_opts& c, int om, const hpo_timing& t) {
+      const int k = R.n_candidates++;
+      const bool blk = use_block_layout(m, c);
+      R.waves_per_block[k] = blk ? 4 : c.waves_per_block;
+      R.blocks_per_cu[k] = (t.grid > 0 && m->n_cu > 0) ? std::max(1, t.grid / m->n_cu) : 0;
+      R.order_mode[k] = om;
+      R.lu_layout[k] = c.lu_layout;
+      R.cell_layout[k] = blk ? 2 : 1;
+      R.sample_ms[k] = t.kernel_ms;
+      if (t.kernel_ms < best_ms) best_ms = t.kernel_ms, best = k;
+    };
+    for (int wi = 0; wi < (block_only ? 1 : 3); ++wi)
       for (int om = 0; om < 3 && R.n_candidates < HPO_TUNE_MAX; ++om) {
         HPO_HIP(hipMemcpyAsync(sorder, sample_orders[om].data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
         hpo_opts c = base;
         c.waves_per_block = wpbs[wi];
         hpo_timing t{};
         launch_integrate(m, S, sy, syo, ns, sP, sdt, sst, nullptr, sorder, nullptr, c, st, &t);
-        const int k = R.n_candidates++;
-        R.waves_per_block[k] = wpbs[wi];
-        R.blocks_per_cu[k] = (t.grid > 0 && m->n_cu > 0) ? std::max(1, t.grid / m->n_cu) : 0;
-        R.order_mode[k] = om;
-        R.lu_layout[k] = c.lu_layout;
-        R.sample_ms[k] = t.kernel_ms;
-        if (t.kernel_ms < best_ms) best_ms = t.kernel_ms, best = k;
+        record(c, om, t);
       }
+    // one workgroup per cell (the block layout) at the best order, where it exists
+    if (block_too && best >= 0 && R.n_candidates < HPO_TUNE_MAX) {
+      const int om = R.order_mode[best];
+      HPO_HIP(hipMemcpyAsync(sorder, sample_orders[om].data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+      hpo_opts c = base;
+      c.cell_layout = 2;
+      hpo_timing t{};
+      launch_integrate(m, S, sy, syo, ns, sP, sdt, sst, nullptr, sorder, nullptr, c, st, &t);
+      record(c, om, t);
+    }
     // FP32 path: the other Newton-matrix layout (explicit inverse / in-place LU) at the best
     // launch shape and order
-    if (o->lu_precision == 1 && best >= 0 && R.n_candidates < HPO_TUNE_MAX) {
+    if (o->lu_precision == 1 && best >= 0 && R.cell_layout[best] == 1 && R.n_candidates < HPO_TUNE_MAX) {
       const int om = R.order_mode[best];
       HPO_HIP(hipMemcpyAsync(sorder, sample_orders[om].data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
       hpo_opts c = base;
       c.waves_per_block = R.waves_per_block[best];
       c.lu_layout = 1 - base.lu_layout;
+      c.cell_layout = 1;
       hpo_timing t{};
       launch_integrate(m, S, sy, syo, ns, sP, sdt, sst, nullptr, sorder, nullptr, c, st, &t);
-      const int k = R.n_candidates++;
-      R.waves_per_block[k] = c.waves_per_block;
-      R.blocks_per_cu[k] = (t.grid > 0 && m->n_cu > 0) ? std::max(1, t.grid / m->n_cu) : 0;
-      R.order_mode[k] = om;
-      R.lu_layout[k] = c.lu_layout;
-      R.sample_ms[k] = t.kernel_ms;
-      if (t.kernel_ms < best_ms) best_ms = t.kernel_ms, best = k;
+      record(c, om, t);
     }
     R.best = best;
     o->waves_per_block = R.waves_per_block[best];
     o->lu_layout = R.lu_layout[best];
+    o->cell_layout = R.cell_layout[best];
     o->blocks_per_cu = 0;   // as many workgroups per CU as fit (what every candidate ran with)
     o->sort_cells = 0;
 

@@ -38,15 +38,19 @@ T* dalloc(size_t n) {
   return static_cast<T*>(p);
 }
 
-// Work-queue counter and per-wave scratch of one launch context. Launches that may run at the
-// same time need separate contexts; a mechanism handle owns one for its own entry points.
+// Work-queue counter and per-wave (wave layout) or per-workgroup (block layout) scratch of one
+// launch context. Launches that may run at the same time need separate contexts; a mechanism
+// handle owns one for its own entry points.
 struct ExecRes {
   uint32_t* d_queue = nullptr;
   double* d_scratch = nullptr;
-  int64_t scratch_slots = 0;    // wave slots the scratch holds
-  int64_t scratch_stride = 0;   // doubles per wave slot
+  int64_t scratch_doubles = 0;  // capacity
   int32_t share_pct = 100;      // largest share of n_CU x workgroups-per-CU the grid may take
 };
+
+// Which kernel layout a launch of this mechanism with these options uses: true = one workgroup
+// per cell (block_kernel.hip), false = one wave per cell.
+bool use_block_layout(const hpo_mech* m, const hpo_opts& o);
 
 void check_opts(const hpo_opts& o);
 
@@ -58,21 +62,25 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
                       const int32_t* d_order, int64_t* d_prof, const hpo_opts& o, hipStream_t st,
                       hpo_timing* tm, const ExecRes* res = nullptr);
 
-// Wave slots a launch on the whole device can use (scratch sizing for explicit contexts).
-int64_t max_wave_slots(hpo_mech* m);
+// Scratch doubles the largest launch on the whole device can need, in either layout (sizing
+// of explicit contexts).
+int64_t max_scratch_doubles(hpo_mech* m);
 
 }  // namespace capi
 }  // namespace hpo
 
 struct hpo_mech {
   hpo::Mechanism host;
-  hpo::DevMechBlob blob;
+  hpo::DevMechBlob blob;        // wave-per-cell tables (has_wave)
+  hpo::DevBlkBlob bblob;        // block-per-cell tables (has_block)
+  bool has_wave = false, has_block = false;
   int device = -1;
   int n_cu = 0;
   uint8_t* d_blob = nullptr;
+  uint8_t* d_bblob = nullptr;
   double* d_scratch = nullptr;
-  int64_t scratch_slots = 0;
-  int64_t scratch_stride = 0;
+  int64_t scratch_cap = 0;      // doubles
+  int64_t scratch_stride = 0;   // doubles per wave slot of the wave layout
   uint32_t* d_queue = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // Launches on the handle's own queue counter and scratch are ordered on the device: each
@@ -85,5 +93,7 @@ struct hpo_mech {
 
   ~hpo_mech();
   void ensure_device();               // uploads the tables on first use (current device)
-  void ensure_scratch(int64_t slots);
+  void ensure_scratch(int64_t slots);  // wave slots of the wave layout
+  void ensure_scratch_doubles(int64_t n);
+  int state_size() const { return host.K(); }
 };

@@ -67,8 +67,10 @@ HPO_HD double fast_log10(double x) { return fast_log(x) * 0.43429448190325182765
 // finite range. The table holds 2^(j/32) correctly rounded (kExp2Tab32; on the device a copy
 // in LDS is passed in). Error: at most 2 ulp against the C library (tests/test_fastmath.py).
 // Results overflow to +inf above 709.78 and underflow through the subnormals to 0 below
-// -745.13, as the C library's do; NaN propagates. Infinite arguments are not handled (they
-// give NaN): the kernels pass finite arguments.
+// -745.13, as the C library's do; NaN propagates. Arguments are first clamped to
+// [-760, 720], where the reduction is exact and the result already 0 or +inf: Troe
+// parameters such as T** = 1e100 or T*** = 1e-30 give arguments of -1e97 or -1e33, and
+// infinite arguments give 0 and +inf.
 constexpr double kExp2Tab32[32] = {
     0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
     0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
@@ -80,6 +82,7 @@ constexpr double kExp2Tab32[32] = {
     0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0};
 
 HPO_HD double tab_exp(double x, const double* tab) {
+  x = x < -760.0 ? -760.0 : (x > 720.0 ? 720.0 : x);   // comparisons with NaN are false
   constexpr double INV_LN2_32 = 0x1.71547652b82fep+5;   // 32 / ln 2
   constexpr double LN2_32_HI = 0x1.62e42ff000000p-6;
   constexpr double LN2_32_LO = -0x1.718432a1b0e26p-40;

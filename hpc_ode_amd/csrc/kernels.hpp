@@ -39,6 +39,7 @@ struct BdfArgs {
   int32_t maxsteps;
   int32_t refine_sweeps;
   int64_t* prof;             // [ncells][HPO_NPROF] shader-clock cycles per phase, or null
+  BlkLayout blay;            // block-per-cell kernel: layout of the block tables at `mech`
 };
 
 struct EvalArgs {
@@ -52,6 +53,7 @@ struct EvalArgs {
   int64_t ncells;
   int64_t ld;
   int32_t mode;
+  BlkLayout blay;            // block-per-cell component kernels: layout of the block tables
 };
 
 struct LuArgs {
@@ -78,6 +80,18 @@ constexpr int kNoallocMaxDevices = 64;
 hipError_t noalloc_scratch(int device, void** ptr);
 inline int bdf_variant(int lu_precision, int lu_layout) { return lu_precision == 0 ? 0 : (lu_layout == 0 ? 1 : 2); }
 hipError_t launch_eval(int ns, const EvalArgs& A, int grid, int block, size_t shmem, hipStream_t st);
+// Block-per-cell kernel (block_kernel.hip): one 256-thread workgroup per cell.
+bool block_kernel_supports(int ns);
+int blk_work_lds_bytes(const BlkLayout& L);           // LDS after the tables, per workgroup
+int64_t blk_scratch_doubles_host(const BlkLayout& L); // global scratch per workgroup
+const void* bdf_block_kernel_fn(int ns, int lu32, int prof);
+hipError_t launch_bdf_block(int ns, const BdfArgs& A, int lu32, int grid, size_t shmem, hipStream_t st);
+// the block kernel's RHS (mode 0) or Jacobian (mode 1) on one workgroup per cell; A.mech holds
+// the block tables, A.scratch grid * blk_scratch_doubles_host doubles
+hipError_t launch_eval_block(int ns, const EvalArgs& A, int grid, size_t shmem, hipStream_t st);
+// the block kernel's Newton-matrix code: Gauss-Jordan inverse across one workgroup per matrix,
+// applied to b (FP64, or FP32 without refinement)
+hipError_t launch_gj_block(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st);
 hipError_t launch_lu(int ns, int fp32, const LuArgs& A, int64_t ncases, hipStream_t st);
 // one 256-thread workgroup per matrix, FP64 (lu_block.hip)
 hipError_t launch_lu_block(int ns, const LuArgs& A, int64_t ncases, hipStream_t st);

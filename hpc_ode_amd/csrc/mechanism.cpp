@@ -766,4 +766,186 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   return blob;
 }
 
+DevBlkBlob build_block_tables(const Mechanism& m) {
+  const int K = m.K(), R = m.R();
+  if (K < 2 || K > 255) throw std::runtime_error("block-per-cell path supports 2..255 species");
+  if (R > 65535) throw std::runtime_error("block-per-cell path supports at most 65535 reactions");
+  DevBlkBlob blob;
+  BlkLayout& L = blob.lay;
+  L.K = K;
+  L.R = R;
+  // device order as on the wave path: Troe, Lindemann, three-body, reversible, irreversible
+  std::vector<const Reaction*> rx_of(R);
+  {
+    std::vector<int> idx(R);
+    for (int r = 0; r < R; ++r) idx[r] = r;
+    auto group = [&](int r) {
+      const Reaction& x = m.reactions[r];
+      if (x.kind == kTroe) return 0;
+      if (x.kind == kLindemann) return 1;
+      if (x.kind == kThreeBody) return 2;
+      return x.reversible ? 3 : 4;
+    };
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return group(a) < group(b); });
+    for (int d = 0; d < R; ++d) rx_of[d] = &m.reactions[idx[d]];
+  }
+  std::vector<std::map<int, int>> net(R);
+  int n_pdep = 0, n_eff = 0;
+  std::vector<int> slot_sp;
+  std::vector<uint16_t> slot_begin(R), slot_count(R);
+  for (int r = 0; r < R; ++r) {
+    const Reaction& rx = *rx_of[r];
+    if (static_cast<int>(rx.reactants.size()) > kBlkSide || static_cast<int>(rx.products.size()) > kBlkSide)
+      throw std::runtime_error("more than 4 species on a reaction side: " + rx.equation);
+    for (auto* side : {&rx.reactants, &rx.products})
+      for (auto& kv : *side)
+        if (kv.second != std::floor(kv.second) || kv.second < 1 || kv.second > 3)
+          throw std::runtime_error("non-integer stoichiometry unsupported: " + rx.equation);
+    for (auto& kv : rx.reactants) net[r][kv.first] -= static_cast<int>(kv.second);
+    for (auto& kv : rx.products) net[r][kv.first] += static_cast<int>(kv.second);
+    slot_begin[r] = static_cast<uint16_t>(slot_sp.size());
+    for (auto& kv : rx.reactants) slot_sp.push_back(kv.first);
+    if (rx.reversible)
+      for (auto& kv : rx.products) slot_sp.push_back(kv.first);
+    if (rx.kind != kElementary) {
+      ++n_pdep;
+      n_eff += static_cast<int>(rx.eff.size());
+      for (auto& kv : rx.eff) slot_sp.push_back(kv.first);
+    }
+    if (slot_sp.size() > 65535) throw std::runtime_error("too many Jacobian slots");
+    slot_count[r] = static_cast<uint16_t>(slot_sp.size() - slot_begin[r]);
+  }
+  L.n_pdep = n_pdep;
+  L.n_eff = n_eff;
+  L.n_slots = static_cast<int>(slot_sp.size());
+  // per-species reaction lists (net coefficient != 0), ascending device reaction index
+  std::vector<uint16_t> csr_ptr(K + 1, 0);
+  std::vector<uint32_t> csr;
+  for (int k = 0; k < K; ++k) {
+    csr_ptr[k] = static_cast<uint16_t>(csr.size());
+    for (int r = 0; r < R; ++r) {
+      auto it = net[r].find(k);
+      if (it == net[r].end() || it->second == 0) continue;
+      if (it->second < -128 || it->second > 127) throw std::runtime_error("net stoichiometric coefficient out of range");
+      csr.push_back(static_cast<uint32_t>(r) | (static_cast<uint32_t>(static_cast<uint8_t>(static_cast<int8_t>(it->second))) << 16));
+    }
+    if (csr.size() > 65535) throw std::runtime_error("too many species-reaction pairs");
+  }
+  csr_ptr[K] = static_cast<uint16_t>(csr.size());
+  L.n_csr = static_cast<int>(csr.size());
+  // per-species Jacobian slots (the temperature row's column sums)
+  std::vector<uint32_t> col_ptr(K + 1, 0);
+  std::vector<uint32_t> col;
+  for (int k = 0; k < K; ++k) {
+    col_ptr[k] = static_cast<uint32_t>(col.size());
+    for (int r = 0; r < R; ++r)
+      for (int q = 0; q < slot_count[r]; ++q)
+        if (slot_sp[slot_begin[r] + q] == k)
+          col.push_back(static_cast<uint32_t>(slot_begin[r] + q) | (static_cast<uint32_t>(r) << 16));
+  }
+  col_ptr[K] = static_cast<uint32_t>(col.size());
+  L.n_col = static_cast<int>(col.size());
+
+  int off = 0;
+  L.off_species = off;
+  off += K * kSpStride * 8;
+  off = align_to(off, 16);
+  L.off_rxn = off;
+  off += R * static_cast<int>(sizeof(BlkRxn));
+  off = align_to(off, 16);
+  L.off_arr = off;
+  off += R * 3 * 8;
+  off = align_to(off, 16);
+  L.off_pdep = off;
+  off += std::max(n_pdep, 1) * static_cast<int>(sizeof(PdepRec));
+  off = align_to(off, 16);
+  L.off_eff_val = off;
+  off += std::max(n_eff, 1) * 8;
+  L.off_eff_sp = off;
+  off += align_to(std::max(n_eff, 1), 16);
+  L.off_csr_ptr = off;
+  off += align_to((K + 1) * 2, 16);
+  L.off_csr = off;
+  off += align_to(std::max(L.n_csr, 1) * 4, 16);
+  L.off_exptab = off;
+  off += 32 * 8;
+  L.lds_bytes = off;
+  L.off_slot_sp = off;
+  off += align_to(std::max(L.n_slots, 1), 16);
+  L.off_col_ptr = off;
+  off += align_to((K + 1) * 4, 16);
+  L.off_col = off;
+  off += align_to(std::max(L.n_col, 1) * 4, 16);
+  L.total_bytes = off;
+
+  blob.bytes.assign(off, 0);
+  uint8_t* B = blob.bytes.data();
+  double* sp = reinterpret_cast<double*>(B + L.off_species);
+  for (int k = 0; k < K; ++k) {
+    sp[k * kSpStride + 0] = 1.0 / m.W[k];
+    sp[k * kSpStride + 1] = m.W[k];
+    sp[k * kSpStride + 2] = m.tmid[k];
+    for (int j = 0; j < 14; ++j) sp[k * kSpStride + 3 + j] = m.nasa[14 * k + j];
+  }
+  BlkRxn* rxn = reinterpret_cast<BlkRxn*>(B + L.off_rxn);
+  double* arr = reinterpret_cast<double*>(B + L.off_arr);
+  PdepRec* pd = reinterpret_cast<PdepRec*>(B + L.off_pdep);
+  double* effv = reinterpret_cast<double*>(B + L.off_eff_val);
+  uint8_t* effs = B + L.off_eff_sp;
+  int ip = 0, ie = 0;
+  for (int r = 0; r < R; ++r) {
+    const Reaction& rx = *rx_of[r];
+    BlkRxn h;
+    std::memset(&h, 0, sizeof(h));
+    for (int q = 0; q < 8; ++q) h.sp[q] = static_cast<uint8_t>(K);
+    for (size_t q = 0; q < rx.reactants.size(); ++q) {
+      h.sp[q] = static_cast<uint8_t>(rx.reactants[q].first);
+      h.nu[q] = static_cast<uint8_t>(rx.reactants[q].second);
+    }
+    for (size_t q = 0; q < rx.products.size(); ++q) {
+      h.sp[kBlkSide + q] = static_cast<uint8_t>(rx.products[q].first);
+      h.nu[kBlkSide + q] = static_cast<uint8_t>(rx.products[q].second);
+    }
+    h.kind = static_cast<uint8_t>(rx.kind);
+    const bool constk = (rx.b == 0.0 && rx.theta == 0.0);
+    h.flags = (rx.reversible ? 1 : 0) | (constk ? 2 : 0) | (rx.troe_has_t2 ? 4 : 0);
+    h.pidx = 0xFFFF;
+    h.slot = slot_begin[r];
+    h.nslot = slot_count[r];
+    arr[3 * r + 0] = constk ? rx.A : rx.lnA;
+    arr[3 * r + 1] = rx.b;
+    arr[3 * r + 2] = rx.theta;
+    if (rx.kind != kElementary) {
+      h.pidx = static_cast<uint16_t>(ip);
+      PdepRec& P = pd[ip];
+      P.eff_base = rx.eff_base;
+      P.eff_begin = ie;
+      for (auto& kv : rx.eff) {
+        effv[ie] = kv.second;
+        effs[ie] = static_cast<uint8_t>(kv.first);
+        ++ie;
+      }
+      P.eff_end = ie;
+      const bool constk0 = (rx.b0 == 0.0 && rx.theta0 == 0.0);
+      if (rx.kind >= kLindemann && constk0) h.flags |= 8;
+      P.lnA0 = (rx.kind >= kLindemann && constk0) ? rx.A0 : rx.lnA0;
+      P.b0 = rx.b0;
+      P.theta0 = rx.theta0;
+      P.troe_a = rx.troe[0];
+      P.troe_iT3 = 1.0 / rx.troe[1];
+      P.troe_iT1 = 1.0 / rx.troe[2];
+      P.troe_T2 = rx.troe[3];
+      ++ip;
+    }
+    rxn[r] = h;
+  }
+  std::memcpy(B + L.off_csr_ptr, csr_ptr.data(), csr_ptr.size() * 2);
+  if (!csr.empty()) std::memcpy(B + L.off_csr, csr.data(), csr.size() * 4);
+  std::memcpy(B + L.off_exptab, kExp2Tab32, sizeof(kExp2Tab32));
+  for (int q = 0; q < L.n_slots; ++q) B[L.off_slot_sp + q] = static_cast<uint8_t>(slot_sp[q]);
+  std::memcpy(B + L.off_col_ptr, col_ptr.data(), col_ptr.size() * 4);
+  if (!col.empty()) std::memcpy(B + L.off_col, col.data(), col.size() * 4);
+  return blob;
+}
+
 }  // namespace hpo

@@ -125,4 +125,53 @@ struct DevMechBlob {
 // non-integer stoichiometry).
 DevMechBlob build_device_tables(const Mechanism& m);
 
+// ---------------------------------------------------------------------------
+// Block-per-cell tables: one workgroup integrates one cell, thread t owning state component t,
+// so the species count is bounded by the workgroup (and 255 by the byte species indices), not
+// by 64 lanes. Species production rates and Jacobian rows are gathered per species from a
+// compressed row list (owner computes, fixed order), the temperature row per column from a
+// list of the Jacobian slots of each species.
+// ---------------------------------------------------------------------------
+constexpr int kBlkSide = 4;     // species per reaction side
+
+struct alignas(8) BlkRxn {      // 24 bytes
+  uint8_t sp[8];                // reactants 0..3, products 4..7; unused slots: the dummy species K
+  uint8_t nu[8];                // integer stoichiometric coefficients (0 in unused slots)
+  uint8_t kind;                 // RxnKind
+  uint8_t flags;                // as RxnHdr::flags
+  uint16_t pidx;                // pressure-dependent record
+  uint16_t slot;                // first d(phi)/dC slot of the reaction
+  uint16_t nslot;               // its slots: reactants, products (reversible), efficiencies
+};
+static_assert(sizeof(BlkRxn) == 24, "BlkRxn layout");
+
+struct BlkLayout {
+  int32_t K, R, n_pdep, n_eff, n_slots, n_csr, n_col;
+  int32_t lds_bytes;            // bytes [0, lds_bytes) are staged to LDS
+  // LDS part
+  int32_t off_species;          // double[K][kSpStride]
+  int32_t off_rxn;              // BlkRxn[R]
+  int32_t off_arr;              // double[R][3]: lnA (or A), b, theta
+  int32_t off_pdep;             // PdepRec[n_pdep]
+  int32_t off_eff_val;          // double[n_eff]
+  int32_t off_eff_sp;           // uint8[n_eff]
+  int32_t off_csr_ptr;          // uint16[K+1]: reactions of species k
+  int32_t off_csr;              // uint32[n_csr]: r | (net coefficient, int8) << 16, ascending r
+  int32_t off_exptab;           // double[32]: 2^(j/32)
+  // global part
+  int32_t off_slot_sp;          // uint8[n_slots]: species of each d(phi)/dC slot
+  int32_t off_col_ptr;          // uint32[K+1]: Jacobian slots whose species is k
+  int32_t off_col;              // uint32[n_col]: slot | reaction << 16
+  int32_t total_bytes;
+};
+
+struct DevBlkBlob {
+  BlkLayout lay{};
+  std::vector<uint8_t> bytes;
+};
+
+// Throws std::runtime_error when the mechanism does not fit (more than 255 species, more than
+// 4 species on a reaction side, non-integer stoichiometry, more than 65535 reactions or slots).
+DevBlkBlob build_block_tables(const Mechanism& m);
+
 }  // namespace hpo

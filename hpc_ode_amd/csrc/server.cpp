@@ -169,7 +169,7 @@ struct hpo_server {
   void launch(int k, const std::vector<SrvReq*>& batch) {
     Slot& sl = slots[k];
     hpo_mech* m = mechs[batch.front()->st->mech];
-    const int64_t ld = m->blob.lay.K;
+    const int64_t ld = m->host.K();
     int64_t total = 0;
     for (SrvReq* r : batch) total += r->st->n;
     const auto now = Clock::now();
@@ -335,16 +335,15 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
     check_opts(s->opts.opts);
     if (s->opts.opts.sort_cells == 2) throw std::invalid_argument("sort_cells = 2 needs a batch");
     HPO_HIP(hipGetDevice(&s->device));
-    int64_t stride = 0, wslots = 0, ldmax = 0;
+    int64_t scratch = 0, ldmax = 0;
     for (int i = 0; i < n_mechs; ++i) {
       if (!mechs[i]) throw std::invalid_argument("null mechanism");
       hpo_mech* m = mechs[i];
-      if (!hpo::kernel_supports(m->blob.lay.K)) throw std::invalid_argument("no kernel for a mechanism's state size");
+      (void)use_block_layout(m, s->opts.opts);   // throws when no kernel fits the state size
       m->ensure_device();
       s->mechs.push_back(m);
-      stride = std::max<int64_t>(stride, m->scratch_stride);
-      wslots = std::max<int64_t>(wslots, max_wave_slots(m));
-      ldmax = std::max<int64_t>(ldmax, m->blob.lay.K);
+      scratch = std::max<int64_t>(scratch, max_scratch_doubles(m));
+      ldmax = std::max<int64_t>(ldmax, m->host.K());
     }
     size_t fr = 0, to = 0;
     HPO_HIP(hipMemGetInfo(&fr, &to));
@@ -356,10 +355,9 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
       HPO_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
       HPO_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
       sl.res.d_queue = dalloc<uint32_t>(4);
-      sl.res.scratch_stride = stride;
-      sl.res.scratch_slots = wslots;
+      sl.res.scratch_doubles = scratch;
       sl.res.share_pct = s->opts.cu_share_pct;
-      sl.res.d_scratch = dalloc<double>(static_cast<size_t>(stride * wslots));
+      sl.res.d_scratch = dalloc<double>(static_cast<size_t>(scratch));
       sl.d_y = dalloc<double>(static_cast<size_t>(sl.cap * ldmax));
       sl.d_P = dalloc<double>(sl.cap);
       sl.d_dt = dalloc<double>(sl.cap);
@@ -395,7 +393,7 @@ int hpo_server_open_stream(hpo_server* s, int32_t mech, int64_t n, const double*
                            double P_scalar, int64_t* id) {
   if (!s || !y0 || !id || n <= 0) return fail(HPO_E_INVALID, "bad argument");
   if (mech < 0 || mech >= static_cast<int32_t>(s->mechs.size())) return fail(HPO_E_INVALID, "no such mechanism");
-  const int64_t K = s->mechs[mech]->blob.lay.K;
+  const int64_t K = s->mechs[mech]->host.K();
   if (ld < K) return fail(HPO_E_INVALID, "ld smaller than the state length");
   if (n > s->opts.max_batch_cells) return fail(HPO_E_INVALID, "stream larger than a launch (max_batch_cells)");
   // y [n][K], then P [n], then status [n], each 256-byte aligned inside one pool block

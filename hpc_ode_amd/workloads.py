@@ -75,3 +75,23 @@ def gri_uneven(species, W, n, seed=0):
     """Wide stiffness spread: T0 U[300,2500] K, phi log-U[0.3,3], dt log-U[1e-6,1e-3] s."""
     return ch4_air(species, W, n, seed=seed, T_range=(300.0, 2500.0), phi_range=(0.3, 3.0),
                    phi_log=True, dt_log_range=(1e-6, 1e-3))
+
+
+def dodecane_air(species, W, n, seed=0, T_range=(1000.0, 1600.0), P_atm_range=(1.0, 20.0),
+                 phi_range=(0.6, 1.6), dt=1e-4):
+    """n-dodecane/air cells for nDodecane_Reitz (100 species): X_C12H26 : X_O2 : X_N2 =
+    phi : 18.5 : 18.5 * 3.76, T0 ~ U(T_range), P ~ U(P_atm_range) atm, phi ~ U(phi_range)."""
+    rng = np.random.default_rng(seed)
+    K = len(species)
+    low = [s.lower() for s in species]
+    W = np.asarray(W)
+    T = rng.uniform(*T_range, n)
+    P = rng.uniform(*P_atm_range, n) * ATM
+    phi = rng.uniform(*phi_range, n)
+    X = np.zeros((n, K))
+    X[:, low.index("c12h26")] = phi
+    X[:, low.index("o2")] = 18.5
+    X[:, low.index("n2")] = 18.5 * 3.76
+    X /= X.sum(axis=1, keepdims=True)
+    y = _to_state(X, W, T)
+    return y, P, np.full(n, dt)

@@ -58,7 +58,10 @@ typedef struct {
                               2 (hpo_batch_integrate only) the order of the last hpo_batch_autotune */
   int32_t lu_layout;       /* with lu_precision == 1: 0 explicit inverse (Gauss-Jordan, solves as
                               matrix-vector products); 1 in-place LU (triangular solves) */
-  int32_t reserved;
+  int32_t cell_layout;     /* 0 automatic (one wave per cell when a wave kernel exists for the state
+                              size, else one workgroup per cell); 1 one wave per cell; 2 one
+                              256-thread workgroup per cell (Gauss-Jordan Newton matrix across the
+                              workgroup; states of more than 64 variables, e.g. nDodecane_Reitz) */
 } hpo_opts;
 
 typedef struct {
@@ -150,6 +153,7 @@ typedef struct {
   int32_t n_candidates, best;
   int32_t waves_per_block[HPO_TUNE_MAX], blocks_per_cu[HPO_TUNE_MAX], order_mode[HPO_TUNE_MAX];
   int32_t lu_layout[HPO_TUNE_MAX];
+  int32_t cell_layout[HPO_TUNE_MAX];   /* 1 one wave per cell, 2 one workgroup per cell */
   double sample_ms[HPO_TUNE_MAX];
   int64_t sample_cells;
   double tune_ms;          /* wall time of the whole tuning call */
@@ -217,10 +221,16 @@ int hpo_server_stream_get(hpo_server *s, int64_t stream_id, double *y, int64_t l
 int hpo_server_hbm_used(const hpo_server *s, int64_t *used_bytes, int64_t *budget_bytes);
 
 /* ---- component entry points (host arrays), used by parity tests ---- */
-/* f = dydt(y) for n states: y, f are [n][n_state]; the pyJac dydt of dydt.c:7-133. */
+/* f = dydt(y) for n states: y, f are [n][n_state]; the pyJac dydt of dydt.c:7-133. Runs the
+ * wave-per-cell code when the mechanism has it (at most 63 species), else the block-per-cell code. */
 int hpo_eval_rhs(const hpo_mech *m, int64_t n, const double *y, const double *P, double *f);
 /* J = d f / d y, [n][n_state][n_state] row-major; what eval_jacob (jacob.c:3-371) computes. */
 int hpo_eval_jac(const hpo_mech *m, int64_t n, const double *y, const double *P, double *J);
+/* hpo_eval_rhs (what = 0) or hpo_eval_jac (what = 1) with the cell layout named as in
+ * hpo_opts::cell_layout (0 auto, 1 wave, 2 block); HPO_E_UNSUPPORTED when the mechanism has no
+ * tables or kernel of that layout. */
+int hpo_eval_layout(const hpo_mech *m, int32_t cell_layout, int32_t what, int64_t n, const double *y,
+                    const double *P, double *out);
 /* Batched dense solve with the kernel's register-resident Newton-matrix code: M [n][ns][ns]
  * row-major, b, x [n][ns]. fp32 = 0: FP64 partial-pivot LU and triangular solves; fp32 = 1: the
  * FP32 Gauss-Jordan inverse (same pivots) applied as a matrix-vector product, no refinement. */
@@ -230,7 +240,9 @@ int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double *M, const dou
  * batched LU (magma_dgetrf_batched / magma_dgetrs_batched via SUNLinSol_MagmaDense,
  * src/Integrators/CVodeIntegratorGPU.cpp:86-91): layout 0 = one wave per matrix (the
  * register code above, fp32 0 or 1); layout 1 = one 256-thread workgroup per matrix, the
- * augmented matrix [M | b] in LDS, FP64 only (fp32 = 1 returns HPO_E_UNSUPPORTED).
+ * augmented matrix [M | b] in LDS, FP64 only (fp32 = 1 returns HPO_E_UNSUPPORTED); layout 2 =
+ * the block-per-cell kernel's Newton-matrix code (ns 53 or 100): Gauss-Jordan inverse with one
+ * matrix row per thread of a 256-thread workgroup, FP64 or FP32, applied to b without refinement.
  * kernel_ms (may be NULL): the launch's device time from HIP events. */
 int hpo_lu_solve_layout(int32_t ns, int32_t fp32, int32_t layout, int64_t n, const double *M,
                         const double *b, double *x, int32_t *ok, float *kernel_ms);

@@ -43,6 +43,14 @@ def strict_match(y: np.ndarray, ref: np.ndarray, rtol: float = RTOL, y_floor: fl
     return dT & np.all((np.abs(Yr) <= y_floor) | (dY <= rtol * np.abs(Yr)), axis=1)
 
 
+def deviation(y: np.ndarray, ref: np.ndarray, rtol: float = RTOL, atol: float = Y_FLOOR):
+    """Per cell: |T - T_ref| / T_ref, and per cell the largest |y - y_ref| / (rtol |y_ref| + atol)
+    over the state's components."""
+    dT = np.abs(y[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])
+    w = (np.abs(y - ref) / (rtol * np.abs(ref) + atol)).max(axis=1)
+    return dT, w
+
+
 def _perturb(name, y0, P, o2_col):
     y, p = y0.copy(), P.copy()
     if name == "T+":
@@ -68,14 +76,17 @@ def roundoff_floor(orc, P, dt, y0, nthreads=0, perturbations=PERTURBATIONS, **kw
     y0 = np.ascontiguousarray(y0, np.float64)
     P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (len(y0),)))
     dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (len(y0),)))
-    o2_col = 1 + list(orc.mech.species).index("O2")
+    o2_col = 1 + [sp.upper() for sp in orc.mech.species].index("O2")
     yref, sref, stref = orc.integrate(P, dt, y0, nthreads=nthreads, **kw)
     match, same_nst = {}, {}
+    spread_T, spread_w = 0.0, 0.0
     for name in perturbations:
         yp, pp = _perturb(name, y0, P, o2_col)
         y, s, st = orc.integrate(pp, dt, yp, nthreads=nthreads, **kw)
         match[name] = strict_match(y, yref) & (s == sref)
         same_nst[name] = st["nst"] == stref["nst"]
+        dT, w = deviation(y, yref)
+        spread_T, spread_w = max(spread_T, float(dT.max())), max(spread_w, float(w.max()))
     M = np.array([match[k] for k in perturbations])
     S = np.array([same_nst[k] for k in perturbations])
     robust = M.all(axis=0)
@@ -92,6 +103,7 @@ def roundoff_floor(orc, P, dt, y0, nthreads=0, perturbations=PERTURBATIONS, **kw
         "same_nst_fraction": float(S.mean()),
         "robust": robust, "robust_fraction": float(robust.mean()),
         "robust_keep_rate": float(min(loo)) if loo else 1.0,
+        "spread_T": spread_T, "spread_w": spread_w,
         "cells": int(len(y0)),
     }
 
@@ -106,6 +118,7 @@ def summary(fl: dict) -> dict:
         "same_nst": fl["same_nst_fraction"],
         "robust_cells": fl["robust_fraction"],
         "robust_keep_rate": fl["robust_keep_rate"],
+        "spread_T": fl["spread_T"], "spread_w": fl["spread_w"],
     }
 
 
@@ -119,6 +132,8 @@ def compare(y, status, stats, fl: dict) -> dict:
         "robust_cells_matched": float(m[rob].mean()) if rob.any() else 1.0,
         "same_nst": float((stats["nst"] == fl["stats"]["nst"]).mean()) if stats is not None else None,
         "floor_same_nst": fl["same_nst_fraction"],
+        "max_dT": float(deviation(y, fl["y"])[0].max()),
+        "max_w": float(deviation(y, fl["y"])[1].max()),
     }
 
 
@@ -143,3 +158,14 @@ def check_against_floor(y, status, stats, fl: dict, sigmas: float = 4.0) -> dict
         smargin = sigmas * math.sqrt(max(s * (1 - s), 0.01) / n)
         assert c["same_nst"] >= s - smargin, f"same step count {c['same_nst']:.3f} < {s:.3f} - {smargin:.3f}"
     return c
+
+
+def check_spread(y, fl: dict, factor: float = 4.0, T_min: float = 1e-5, w_min: float = 300.0):
+    """Every cell, matched or not: |dT| / T and the largest |dy| / (rtol |y| + atol) stay within
+    `factor` times the largest deviation the one-ulp runs of the floor showed (and never below
+    T_min, w_min). A sanity bound on the cells the strict criterion does not judge."""
+    dT, w = deviation(y, fl["y"])
+    bT, bw = max(T_min, factor * fl["spread_T"]), max(w_min, factor * fl["spread_w"])
+    assert dT.max() <= bT, f"T max rel diff {dT.max():.3e} > {bT:.3e} (floor spread {fl['spread_T']:.3e})"
+    assert w.max() <= bw, f"max |dy|/(rtol|y|+atol) {w.max():.3e} > {bw:.3e} (floor spread {fl['spread_w']:.3e})"
+    return dT, w

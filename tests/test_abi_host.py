@@ -71,9 +71,21 @@ def test_invalid_arguments_return_error_codes():
     args = [M.ctypes.data_as(C.POINTER(C.c_double)), b.ctypes.data_as(C.POINTER(C.c_double)),
             x.ctypes.data_as(C.POINTER(C.c_double)), ok.ctypes.data_as(C.POINTER(C.c_int32)), None]
     assert L.hpo_lu_solve_layout(10, 0, 1, 1, None, None, None, None, None) == -1
-    assert L.hpo_lu_solve_layout(10, 0, 2, 1, *args) == -1
+    assert L.hpo_lu_solve_layout(10, 0, 3, 1, *args) == -1
     assert L.hpo_lu_solve_layout(10, 1, 1, 1, *args) == -5
     assert L.hpo_lu_solve_layout(7, 0, 1, 1, *args) == -5
+    assert L.hpo_lu_solve_layout(10, 0, 2, 1, *args) == -5   # the block code exists for 53 and 100
+
+
+def test_eval_layout_checks_come_before_device_calls(h2_path):
+    m = H.Mechanism(h2_path)
+    L = H.lib()
+    y, P, out = np.zeros((1, m.n_state)), np.ones(1), np.zeros((1, m.n_state))
+    dp = C.POINTER(C.c_double)
+    a = [y.ctypes.data_as(dp), P.ctypes.data_as(dp), out.ctypes.data_as(dp)]
+    assert L.hpo_eval_layout(m.handle, 3, 0, 1, *a) == -1
+    assert L.hpo_eval_layout(m.handle, 1, 2, 1, *a) == -1
+    assert L.hpo_eval_layout(m.handle, 2, 0, 1, *a) == -5   # no block kernel for 10 variables
 
 
 def test_server_option_checks_come_before_device_calls(h2_path):

@@ -61,3 +61,6 @@ def test_exp_special_values(results):
     assert results["exp_big"] == float("inf") and results["exp_tiny"] == 0.0
     assert results["exp_nan"] != results["exp_nan"]
     assert 1.79e308 < results["exp_max"] < float("inf")
+    # huge finite arguments (a Troe term exp(-T / 1e-30) or exp(-1e100 / T)) and infinities
+    assert results["exp_neg_huge"] == 0.0 and results["exp_neginf"] == 0.0
+    assert results["exp_pos_huge"] == float("inf") and results["exp_posinf"] == float("inf")

@@ -47,10 +47,16 @@ def test_autotune_order_and_shape_do_not_change_results(gri):
     b.put(y0, P, dt)
     o = H.opts(lu_precision=1, refine_sweeps=1)
     rep = b.autotune(n, o, sample=1024)
-    # 3 launch shapes x 3 queue orders, then the other FP32 Newton-matrix layout at the best
-    assert rep.n_candidates == 10 and 0 <= rep.best < 10 and rep.sample_cells == 1024
-    assert [c["lu_layout"] for c in rep.candidates()] == ["inverse"] * 9 + ["lu"]
+    # 3 launch shapes x 3 queue orders of the wave-per-cell kernel, the block-per-cell kernel at
+    # the best order, then (when a wave layout wins) the other FP32 Newton-matrix layout
+    assert rep.n_candidates in (10, 11) and 0 <= rep.best < rep.n_candidates and rep.sample_cells == 1024
+    cands = rep.candidates()
+    assert [c["cell_layout"] for c in cands[:10]] == ["wave"] * 9 + ["block"]
+    assert [c["lu_layout"] for c in cands[:10]] == ["inverse"] * 10
+    if rep.n_candidates == 11:
+        assert cands[10]["cell_layout"] == "wave" and cands[10]["lu_layout"] == "lu"
     assert o.waves_per_block in (1, 2, 4) and o.sort_cells == 2 and o.lu_layout in (0, 1)
+    assert o.cell_layout in (1, 2)
     assert min(c["sample_ms"] for c in rep.candidates()) == rep.candidates()[rep.best]["sample_ms"]
     # tuning integrates a copy: the resident cells are unchanged
     y_before, _, _ = b.get(n)
@@ -62,9 +68,10 @@ def test_autotune_order_and_shape_do_not_change_results(gri):
         return b.get(n)
 
     ya, sa, ca = run(o)
-    yb, sb, cb = run(H.opts(lu_precision=1, refine_sweeps=1, waves_per_block=4, lu_layout=o.lu_layout))
+    yb, sb, cb = run(H.opts(lu_precision=1, refine_sweeps=1, waves_per_block=4, lu_layout=o.lu_layout,
+                            cell_layout=o.cell_layout))
     yc, sc, cc = run(H.opts(lu_precision=1, refine_sweeps=1, waves_per_block=1, sort_cells=1,
-                            lu_layout=o.lu_layout))
+                            lu_layout=o.lu_layout, cell_layout=o.cell_layout))
     assert (sa == 0).all()
     assert np.array_equal(ya, yb) and np.array_equal(ya, yc)
     for f in H.STATS_FIELDS:

@@ -213,3 +213,56 @@ def test_h2li_xml_mechanism_jacobian_and_conservation(h2li_path):
         assert _check_fd(o, 101325.0, yy, 1e-5) < 1e-5
         w = o.rates(101325.0, y)["wdot"]
         assert abs((w * o.mech.W).sum()) <= 1e-11 * np.abs(w * o.mech.W).max() + 1e-300
+
+
+@pytest.fixture(scope="module")
+def dodecane():
+    return O.Oracle(os.path.join(os.path.dirname(os.path.dirname(__file__)), "hpc_ode_amd", "mechanisms",
+                                 "nDodecane_Reitz.yaml"))
+
+
+def test_ndodecane_tables(dodecane):
+    """nDodecane_Reitz (the reference's resources/mechanisms, 100 species, 553 reactions):
+    reactions with four product species and the eliminated last species A4."""
+    t = dodecane.t
+    assert (dodecane.K, dodecane.R) == (100, 553)
+    assert t.nprod.max() == 4 and t.nreac.max() <= 4
+    assert dodecane.mech.species[-1] == "A4"
+
+
+@pytest.mark.parametrize("T", [800.0, 1100.0, 1500.0, 2200.0])
+def test_ndodecane_jacobian_central_differences(dodecane, T):
+    """The analytic Jacobian of the 100-species mechanism (what the block-per-cell kernel
+    reproduces) matches central differences of the RHS."""
+    m = dodecane.mech
+    rng = np.random.default_rng(int(T))
+    y, P, _ = WL.dodecane_air(m.species, m.W, 1, seed=int(T))
+    y = y[0]
+    y[0] = T
+    for s in ("h", "o", "oh", "ho2", "ch3", "h2o", "co", "c2h4", "ch2o"):
+        y[1 + m.species.index(s)] = rng.uniform(1e-4, 1e-2)
+    if min(abs(T - tm) for tm in set(m.tmid.tolist())) < 2.0:
+        pytest.skip("central differences would straddle a NASA7 range switch")
+    assert _check_fd(dodecane, P[0], y, 1e-5) < 1e-5
+
+
+def test_ndodecane_mass_and_element_conservation(dodecane):
+    m = dodecane.mech
+    import yaml
+    from oracle.mech import _Loader
+    with open(os.path.join(os.path.dirname(os.path.dirname(__file__)), "hpc_ode_amd", "mechanisms",
+                           "nDodecane_Reitz.yaml")) as f:
+        doc = yaml.load(f, Loader=_Loader)
+    comp = {s["name"]: s["composition"] for s in doc["species"]}
+    elements = sorted({e for c in comp.values() for e in c})
+    E = np.array([[comp[s].get(e, 0) for e in elements] for s in m.species], float)
+    y, P, _ = WL.dodecane_air(m.species, m.W, 3, seed=5)
+    for i in range(3):
+        yy = y[i].copy()
+        yy[0] = 1400.0
+        for s in ("h", "o", "oh", "ho2", "ch3", "co", "c2h4"):
+            yy[1 + m.species.index(s)] = 1e-3
+        w = dodecane.rates(P[i], yy)["wdot"]
+        scale = np.abs(w * m.W).max()
+        assert abs((w * m.W).sum()) < 1e-11 * scale
+        assert np.all(np.abs(E.T @ w) < 1e-11 * np.abs(w).max() * 10)

@@ -70,3 +70,16 @@ def test_h2_floor(h2_path):
     assert 0.0 <= fl["fraction"] <= 1.0 and fl["robust_keep_rate"] >= 0.9
     y, s, st = o.integrate(P, dt, np.nextafter(y0, 2.0), nthreads=8)
     F.check_against_floor(y, s, st, fl)
+
+
+def test_spread_bound(gri_floor):
+    """The sanity bound on every cell (check_spread): a one-ulp run stays inside it; a run at a
+    100 times looser tolerance, or with the pressure off by 1e-4, does not."""
+    o, y0, P, dt, fl = gri_floor
+    assert 0.0 < fl["spread_T"] < 1e-4 and fl["spread_w"] > 1.0
+    y, s, st = o.integrate(P, dt, np.nextafter(y0, 0.0), nthreads=8)
+    F.check_spread(y, fl)
+    for kw, Pm in ((dict(rtol=1e-4), P), ({}, P * (1.0 + 1e-4))):
+        y, s, st = o.integrate(Pm, dt, y0, nthreads=8, **kw)
+        with pytest.raises(AssertionError):
+            F.check_spread(y, fl)
