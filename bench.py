@@ -57,6 +57,9 @@ def parse():
                     help="serving: largest share of the device one launch may take, in percent (0 = 100)")
     ap.add_argument("--serve-loads", default="0.3,0.7",
                     help="offered loads as fractions of the measured serving capacity")
+    ap.add_argument("--serve-substeps", type=int, default=1000,
+                    help="serving: each load runs until it has offered at least this many substeps")
+    ap.add_argument("--serve-slots", type=int, default=4, help="serving: launch slots (1..16)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -77,14 +80,15 @@ def measured_traffic(workload, ncells, lu_precision):
     return rec.get("traffic_bytes")
 
 
-def serving_measurement(H, gri, seconds, fractions, cu_share=0):
+def serving_measurement(H, gri, seconds, fractions, cu_share=0, min_substeps=1000, slots=4):
     """p50/p99 substep latency of the serving loop at offered loads stated as fractions of its
     measured capacity (requests: 70 % H2/air, 30 % GRI-3.0, log-uniform [1, 65536] cells,
-    dt U[1e-5, 1e-4] s, two substeps per request stream, Poisson arrivals)."""
+    dt U[1e-5, 1e-4] s, two substeps per request stream, Poisson arrivals), each load offered
+    for its share of `seconds` or until it has offered min_substeps substeps."""
     from hpc_ode_amd import serving as SV
     h2 = H.Mechanism(os.path.join(ROOT, "hpc_ode_amd", "mechanisms", "h2o2.yaml"))
     mechs = [h2, gri]
-    srv = H.Server(mechs, n_slots=4, cu_share_pct=cu_share)
+    srv = H.Server(mechs, n_slots=slots, cu_share_pct=cu_share)
     try:
         cap = SV.capacity(srv, mechs)
         mc = SV.mean_cells()
@@ -93,10 +97,10 @@ def serving_measurement(H, gri, seconds, fractions, cu_share=0):
         for f in fractions:
             rate = f * cap / (mc * substeps)
             r = SV.run_load(srv, mechs, rate, seconds / len(fractions), seed=11, substeps=substeps,
-                            log=lambda m: print(m, file=sys.stderr, flush=True))
+                            log=lambda m: print(m, file=sys.stderr, flush=True), min_substeps=min_substeps)
             r["fraction_of_capacity"] = f
             loads.append(r)
-        return {"capacity_cells_per_s": cap, "mean_cells_per_request": mc, "slots": 4,
+        return {"capacity_cells_per_s": cap, "mean_cells_per_request": mc, "slots": slots,
                 "cu_share_pct": cu_share or 100,
                 "mix": "70% H2/air (h2o2.yaml), 30% GRI-3.0; cells log-U[1,65536]; dt U[1e-5,1e-4] s; "
                        "2 substeps per stream with state resident between them; Poisson arrivals",
@@ -369,7 +373,7 @@ def main():
     serving = None
     if a.serve_seconds > 0:
         serving = serving_measurement(H, mech, a.serve_seconds, [float(x) for x in a.serve_loads.split(",")],
-                                      a.cu_share)
+                                      a.cu_share, a.serve_substeps, a.serve_slots)
 
     value = n / (ms32 / 1e3)
     roof = None

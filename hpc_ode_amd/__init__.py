@@ -35,7 +35,8 @@ class Opts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("maxsteps", C.c_int32),
                 ("lu_precision", C.c_int32), ("refine_sweeps", C.c_int32),
                 ("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("sort_cells", C.c_int32), ("lu_layout", C.c_int32), ("cell_layout", C.c_int32)]
+                ("sort_cells", C.c_int32), ("lu_layout", C.c_int32), ("cell_layout", C.c_int32),
+                ("reactor", C.c_int32)]
 
 
 class Timing(C.Structure):
@@ -112,9 +113,10 @@ def lib():
                                            C.POINTER(Opts), vp, C.POINTER(Timing)]
         L.hpo_integrate_device_noalloc.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, vp, vp, vp,
                                                    C.POINTER(Opts), vp]
+        L.hpo_mech_enthalpy_mass.argtypes = [vp, C.c_int64, _dp, _dp, C.c_int64, _dp]
         L.hpo_eval_rhs.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
         L.hpo_eval_jac.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
-        L.hpo_eval_layout.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp]
+        L.hpo_eval_layout.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp]
         L.hpo_lu_solve.argtypes = [C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip]
         L.hpo_lu_solve_layout.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _ip,
                                           C.POINTER(C.c_float)]
@@ -148,8 +150,20 @@ def _p(a, t=_dp):
 CELL_LAYOUTS = {0: "auto", 1: "wave", 2: "block"}
 
 
+REACTORS = {"CONP": 0, "CONV": 1}
+
+
+def _reactor(r):
+    """0/1 or the reference's names: "CONP" constant pressure, "CONV" constant volume."""
+    if isinstance(r, str):
+        if r not in REACTORS:
+            raise ValueError(f"reactor must be one of {sorted(REACTORS)}")
+        return REACTORS[r]
+    return int(r)
+
+
 def opts(rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0, refine_sweeps=1,
-         waves_per_block=0, blocks_per_cu=0, sort_cells=0, lu_layout=0, cell_layout=0) -> Opts:
+         waves_per_block=0, blocks_per_cu=0, sort_cells=0, lu_layout=0, cell_layout=0, reactor=0) -> Opts:
     o = Opts()
     lib().hpo_opts_default(C.byref(o))
     o.rtol, o.atol, o.maxsteps = rtol, atol, maxsteps
@@ -157,6 +171,7 @@ def opts(rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0, refine_sweeps=1,
     o.waves_per_block, o.blocks_per_cu, o.sort_cells = waves_per_block, blocks_per_cu, sort_cells
     o.lu_layout = lu_layout
     o.cell_layout = cell_layout
+    o.reactor = _reactor(reactor)
     return o
 
 
@@ -200,43 +215,64 @@ class Mechanism:
         _check(lib().hpo_mech_tables(self._h, _p(nasa), _p(tmid), _p(arr), _p(kind, _ip), _p(rev, _ip)))
         return dict(nasa=nasa, tmid=tmid, arr=arr, kind=kind, rev=rev)
 
+    def enthalpy_mass(self, T, Y):
+        """h(T, Y) in J/kg for temperatures T[n] and full mass-fraction rows Y[n, n_species]."""
+        T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
+        Y = np.ascontiguousarray(np.atleast_2d(Y), np.float64)
+        if Y.shape != (len(T), self.n_species):
+            raise ValueError("Y must be [len(T), n_species]")
+        h = np.zeros(len(T))
+        _check(lib().hpo_mech_enthalpy_mass(self._h, len(T), _p(T), _p(Y), Y.shape[1], _p(h)))
+        return h
+
     # --- component evaluations (GPU) -------------------------------------
-    def rhs(self, y, P, cell_layout=0):
-        """dydt on the GPU; cell_layout as in integrate (0 auto, 1 wave, 2 block)."""
+    def rhs(self, y, P, cell_layout=0, reactor=0):
+        """dydt on the GPU; cell_layout as in integrate (0 auto, 1 wave, 2 block); reactor "CONP"
+        (P = pressure, Pa) or "CONV" (P = density, kg/m^3)."""
         y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
         n = y.shape[0]
         P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
         f = np.zeros_like(y)
-        if cell_layout == 0:
+        r = _reactor(reactor)
+        if cell_layout == 0 and r == 0:
             _check(lib().hpo_eval_rhs(self._h, n, _p(y), _p(P), _p(f)))
         else:
-            _check(lib().hpo_eval_layout(self._h, cell_layout, 0, n, _p(y), _p(P), _p(f)))
+            _check(lib().hpo_eval_layout(self._h, cell_layout, r, 0, n, _p(y), _p(P), _p(f)))
         return f
 
-    def jac(self, y, P, cell_layout=0):
-        """d dydt / dy on the GPU, [n, n_state, n_state]; cell_layout as in rhs."""
+    def jac(self, y, P, cell_layout=0, reactor=0):
+        """d dydt / dy on the GPU, [n, n_state, n_state]; cell_layout and reactor as in rhs."""
         y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
         n = y.shape[0]
         P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
         J = np.zeros((n, self.n_state, self.n_state))
-        if cell_layout == 0:
+        r = _reactor(reactor)
+        if cell_layout == 0 and r == 0:
             _check(lib().hpo_eval_jac(self._h, n, _p(y), _p(P), _p(J)))
         else:
-            _check(lib().hpo_eval_layout(self._h, cell_layout, 1, n, _p(y), _p(P), _p(J)))
+            _check(lib().hpo_eval_layout(self._h, cell_layout, r, 1, n, _p(y), _p(P), _p(J)))
         return J
+
+    def density(self, y, P):
+        """rho = P Wbar / (R T) of states y [n, n_state] at pressures P (kg/m^3): the value a
+        constant-volume cell that starts at (T, Y, P) keeps (reactor "CONV" takes it as P)."""
+        y = np.atleast_2d(np.asarray(y, np.float64))
+        S = (y[:, 1:] / self.W[:-1]).sum(axis=1) + (1.0 - y[:, 1:].sum(axis=1)) / self.W[-1]
+        return np.asarray(P, np.float64) / (8314.4621 * y[:, 0] * S)
 
     # --- integration ------------------------------------------------------
     def integrate(self, y, P, dt, rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0,
                   refine_sweeps=1, waves_per_block=0, blocks_per_cu=0, sort_cells=0,
-                  return_timing=False, lu_layout=0, cell_layout=0):
-        """Integrates every row of y ([ncells, n_state]) from t=0 to dt on the GPU.
+                  return_timing=False, lu_layout=0, cell_layout=0, reactor=0):
+        """Integrates every row of y ([ncells, n_state]) from t=0 to dt on the GPU. reactor
+        "CONP": P is each cell's pressure; "CONV": P is each cell's density (see density()).
         Returns (y_out, status, stats[, timing])."""
         y = np.ascontiguousarray(np.atleast_2d(y), np.float64)
         n = y.shape[0]
         b = Batch(self, n)
         b.put(y, P, dt)
         t = b.integrate(n, opts(rtol, atol, maxsteps, lu_precision, refine_sweeps, waves_per_block,
-                                blocks_per_cu, sort_cells, lu_layout, cell_layout))
+                                blocks_per_cu, sort_cells, lu_layout, cell_layout, reactor))
         yo, st, stats = b.get(n)
         return (yo, st, stats, t) if return_timing else (yo, st, stats)
 

@@ -144,6 +144,8 @@ struct Tabs {
   const uint64_t* jterm;      // [njl][64] (global)
   const double* exptab;       // [32] 2^(j/32) for tab_exp
   int K, R, ngl, njl, npart, neffp, npdep, mol_ok;
+  int conv;                   // reactor: 0 constant pressure (P is the pressure), 1 constant
+                              // volume (P is the density)
 };
 
 HPO_DI Tabs make_tabs(const uint8_t* lds, const uint8_t* glob, const DevMechLayout& L) {
@@ -171,6 +173,7 @@ HPO_DI Tabs make_tabs(const uint8_t* lds, const uint8_t* glob, const DevMechLayo
   t.ngl = L.n_gather_lane;
   t.njl = L.n_jterm_lane;
   t.npart = L.n_partials;
+  t.conv = 0;
   return t;
 }
 
@@ -248,7 +251,9 @@ HPO_DI void species_state(const Tabs& t, double y, double P, int lane_in, SpStat
   s.Yk = Yk;
   double S = yw + ylast * t.sp[(K - 1) * kSpStride];
   s.Wbar = 1.0 / S;
-  s.m = P * s.invT * (1.0 / kR);   // P / (R T)
+  // total concentration: P / (R T) at constant pressure; at constant volume P holds the
+  // density and m = rho / Wbar
+  s.m = t.conv ? P * S : P * s.invT * (1.0 / kR);
   s.rho = s.m * s.Wbar;
   s.invrho = S / s.m;
   s.lnPfac = kLnPatmOverR - s.lnT;   // log(P_atm / (R T))
@@ -449,9 +454,11 @@ HPO_DI double rhs(const Tabs& t, const WaveLds& w, double y, double P, int lane_
   double wd[1];
   gather<1>(t, src, w.part, lane, wd);
   const double wdot = wd[0];
-  const double Hk = s.hRT * kR * s.T;  // molar enthalpy
+  // constant pressure: molar enthalpy and cp; constant volume: internal energy H - R T and cv
+  const double cvf = t.conv;
+  const double Hk = (s.hRT - cvf) * kR * s.T;
   double Q = wsum(lane < K ? Hk * wdot : 0.0);
-  double cpmix = wsum(lane < K ? s.Yk * s.cpR * kR * s.invW : 0.0);
+  double cpmix = wsum(lane < K ? s.Yk * (s.cpR - cvf) * kR * s.invW : 0.0);
   double fY = wdot * s.W * s.invrho;
   double f = shfl(fY, (lane + 63) & 63);
   if (lane == 0) f = -Q * s.invrho / cpmix;
@@ -470,7 +477,9 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
   const int K = t.K, R = t.R;
   SpState s;
   species_state(t, y, P, lane, s);
-  const double cpk = s.cpR * kR * s.invW;  // mass cp of species k
+  // constant volume replaces enthalpies by internal energies (H - R T) and cp by cv
+  const double cvf = t.conv;
+  const double cpk = (s.cpR - cvf) * kR * s.invW;  // mass cp (cv) of species k
   if (lane < K) {
     w.C[lane] = s.rho * s.Yk * s.invW;
     w.g[lane] = s.hRT - s.sR;
@@ -519,7 +528,7 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
           dHRT -= h.nu[q] * w.hRT[h.sp[q]];
           dnu -= h.nu[q];
         }
-      dH = dHRT * RT;
+      dH = (dHRT - cvf * dnu) * RT;
       if (h.flags & 1) {
         kr = kf * tab_exp(dG - dnu * s.lnPfac, t.exptab);
         rev = kr;
@@ -642,10 +651,10 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
   double gv[3];
   gather<3>(t, srcs, w.part, lane, gv);  // contains wave_sync
   const double wdot = gv[0], JT = gv[1], Jm = gv[2];
-  const double Hk = s.hRT * RT;
+  const double Hk = (s.hRT - cvf) * RT;
   const bool sp = lane < K;
   const double Q = wsum(sp ? Hk * wdot : 0.0);
-  const double SCpw = wsum(sp ? s.cpR * kR * wdot : 0.0);
+  const double SCpw = wsum(sp ? (s.cpR - cvf) * kR * wdot : 0.0);
   const double SHJT = wsum(sp ? Hk * JT : 0.0);
   const double SHJm = wsum(sp ? Hk * Jm : 0.0);
   const double cpmix = wsum(sp ? s.Yk * cpk : 0.0);
@@ -670,23 +679,32 @@ HPO_DI void jacobian(const Tabs& t, const WaveLds& w, const WaveScratch& g, doub
   const double invD = 1.0 / D;
   const double fT = -Q * invD;
   const double cpN = w.cp[K - 1];
+  // Chain rule to (T, Y). Constant pressure: C_i = P Y_i / (R T Wbar W_i) and m = P / (R T)
+  // move with T (-C/T, -m/T) and C with Y through Wbar (-Wbar d_j C_i), and the species rows'
+  // 1/rho moves too (+wdot/T, +wdot Wbar d_j). Constant volume: C_i = rho Y_i / W_i does not
+  // depend on T, m = rho sum Y/W gives dm/dY_j = rho d_j, and rho is fixed.
+  const double cpf = 1.0 - cvf;
   // column 0 (temperature)
   {
-    double dWdT = JT_k - sigma * s.invT - (s.m * s.invT) * Jm_k;
-    double spv = fac * (dWdT + wdot_k * s.invT);
-    double dQdT = SCpw + SHJT - sigma * s.invT - (s.m * s.invT) * SHJm;  // lane 0: sigma = sum_k H_k sigma_k
-    double dDdT = -rho * cpmix * s.invT + rho * dcpmix;
+    double dWdT = JT_k - cpf * (sigma * s.invT + (s.m * s.invT) * Jm_k);
+    double spv = fac * (dWdT + cpf * wdot_k * s.invT);
+    double dQdT = SCpw + SHJT - cpf * (sigma * s.invT + (s.m * s.invT) * SHJm);  // lane 0: sigma = sum_k H_k sigma_k
+    double dDdT = -cpf * rho * cpmix * s.invT + rho * dcpmix;
     double tv = (-dQdT - fT * dDdT) * invD;
     if (live) g.J[lane] = isT ? tv : spv;
   }
   const double c1 = jcN * rho * invWN;
+  // per column j: base += d_j * ysig, species rows add d_j * wcor, D's derivative d_j * dcoef
+  const double ysig = t.conv ? rho * (isT ? SHJm : Jm_k) : -Wbar * sigma;
+  const double wcor = cpf * wdot_k * Wbar;
+  const double dcoef = -cpf * rho * Wbar * cpmix;
 #pragma unroll 2
   for (int j = 0; j < NS - 1; ++j) {
     const double invWj = t.sp[j * kSpStride];
     const double dj = invWj - invWN;
-    const double base = jrow[j * 64] * rho * invWj - c1 - Wbar * dj * sigma;
-    double spv = fac * (base + wdot_k * Wbar * dj);
-    double dDdY = -rho * Wbar * dj * cpmix + rho * (w.cp[j] - cpN);
+    const double base = jrow[j * 64] * rho * invWj - c1 + dj * ysig;
+    double spv = fac * (base + wcor * dj);
+    double dDdY = dcoef * dj + rho * (w.cp[j] - cpN);
     double tv = (-base - fT * dDdY) * invD;
     if (live) g.J[(j + 1) * NS + lane] = isT ? tv : spv;
   }

@@ -168,7 +168,9 @@ constexpr int bdf_waves_per_eu() {
   return NS <= 16 ? HPO_WAVES_PER_EU_SMALL : (LU32 ? HPO_WAVES_PER_EU_FP32 : HPO_WAVES_PER_EU_FP64);
 }
 
-template <int NS, bool LU32, bool PROF, bool INV>
+// CONV: the reactor as a compile-time constant (constant volume), so the constant-pressure
+// instantiation carries none of the other reactor's selects.
+template <int NS, bool LU32, bool PROF, bool INV, bool CONV>
 __global__ __launch_bounds__(256) __attribute__((
     amdgpu_waves_per_eu(bdf_waves_per_eu<NS, LU32>(), bdf_waves_per_eu<NS, LU32>()))) void bdf_kernel(BdfArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -180,6 +182,7 @@ __global__ __launch_bounds__(256) __attribute__((
   const int wave = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
   Tabs t = make_tabs(smem, A.mech, Lay);
+  t.conv = CONV ? 1 : 0;
   const int wl_doubles = wave_lds_doubles(Lay.R, Lay.n_partials) + kScalDoubles;
   double* wbase = reinterpret_cast<double*>(smem + Lay.lds_bytes) + wave * wl_doubles;
   WaveLds w = make_wave_lds(wbase, Lay.R, Lay.n_partials);
@@ -211,6 +214,7 @@ __global__ __launch_bounds__(256) void eval_kernel(EvalArgs A) {
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
   Tabs t = make_tabs(smem, A.mech, Lay);
+  t.conv = A.reactor;
   const int wl_doubles = wave_lds_doubles(Lay.R, Lay.n_partials);
   WaveLds w = make_wave_lds(reinterpret_cast<double*>(smem + Lay.lds_bytes) + wave * wl_doubles, Lay.R, Lay.n_partials);
   const int64_t slot = static_cast<int64_t>(blockIdx.x) * wpb + wave;
@@ -258,34 +262,40 @@ __global__ __launch_bounds__(64) void lu_kernel(LuArgs A) {
 // launch wrappers (host)
 // ---------------------------------------------------------------------------
 // Kernel variants: 0 FP64 LU; 1 FP32 explicit inverse (Gauss-Jordan); 2 FP32 in-place LU.
-template <int NS, bool PROF>
+template <int NS, bool PROF, bool CONV>
 static const void* bdf_fn_p(int variant) {
-  if (variant == 1) return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, PROF, true>);
-  if (variant == 2) return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, PROF, false>);
-  return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false, PROF, false>);
+  if (variant == 1) return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, PROF, true, CONV>);
+  if (variant == 2) return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, true, PROF, false, CONV>);
+  return reinterpret_cast<const void*>(&dev::bdf_kernel<NS, false, PROF, false, CONV>);
 }
 
 template <int NS>
-static const void* bdf_fn(int variant, int prof) {
-  return prof ? bdf_fn_p<NS, true>(variant) : bdf_fn_p<NS, false>(variant);
+static const void* bdf_fn(int variant, int prof, int conv) {
+  if (conv) return prof ? bdf_fn_p<NS, true, true>(variant) : bdf_fn_p<NS, false, true>(variant);
+  return prof ? bdf_fn_p<NS, true, false>(variant) : bdf_fn_p<NS, false, false>(variant);
 }
 
-template <int NS, bool PROF>
+template <int NS, bool PROF, bool CONV>
 static void launch_bdf_p(const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st) {
   if (variant == 1)
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, PROF, true>), dim3(grid), dim3(block), shmem, st, A);
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, PROF, true, CONV>), dim3(grid), dim3(block), shmem, st, A);
   else if (variant == 2)
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, PROF, false>), dim3(grid), dim3(block), shmem, st, A);
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, true, PROF, false, CONV>), dim3(grid), dim3(block), shmem, st, A);
   else
-    hipLaunchKernelGGL((dev::bdf_kernel<NS, false, PROF, false>), dim3(grid), dim3(block), shmem, st, A);
+    hipLaunchKernelGGL((dev::bdf_kernel<NS, false, PROF, false, CONV>), dim3(grid), dim3(block), shmem, st, A);
 }
 
 template <int NS>
 static hipError_t launch_bdf_ns(const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st) {
-  if (A.prof != nullptr)
-    launch_bdf_p<NS, true>(A, variant, grid, block, shmem, st);
+  const bool prof = A.prof != nullptr, conv = A.reactor == 1;
+  if (prof && conv)
+    launch_bdf_p<NS, true, true>(A, variant, grid, block, shmem, st);
+  else if (prof)
+    launch_bdf_p<NS, true, false>(A, variant, grid, block, shmem, st);
+  else if (conv)
+    launch_bdf_p<NS, false, true>(A, variant, grid, block, shmem, st);
   else
-    launch_bdf_p<NS, false>(A, variant, grid, block, shmem, st);
+    launch_bdf_p<NS, false, false>(A, variant, grid, block, shmem, st);
   return hipGetLastError();
 }
 
@@ -303,9 +313,9 @@ bool kernel_supports(int ns) {
   return false;
 }
 
-const void* bdf_kernel_fn(int ns, int variant, int prof) {
+const void* bdf_kernel_fn(int ns, int variant, int prof, int conv) {
 #define HPO_CASE(N) \
-  if (ns == N) return bdf_fn<N>(variant, prof);
+  if (ns == N) return bdf_fn<N>(variant, prof, conv);
   HPO_FOR_EACH_NS(HPO_CASE)
 #undef HPO_CASE
   return nullptr;

@@ -46,6 +46,8 @@ struct BlkTabs {
   const uint32_t* col_ptr; // global
   const uint32_t* col;     // global
   int K, R;
+  int conv;                // reactor: 0 constant pressure (P is the pressure), 1 constant volume
+                           // (P is the density)
 };
 
 HPO_DI BlkTabs make_blk_tabs(const uint8_t* lds, const uint8_t* glob, const BlkLayout& L) {
@@ -64,6 +66,7 @@ HPO_DI BlkTabs make_blk_tabs(const uint8_t* lds, const uint8_t* glob, const BlkL
   t.col = reinterpret_cast<const uint32_t*>(glob + L.off_col);
   t.K = L.K;
   t.R = L.R;
+  t.conv = 0;
   return t;
 }
 
@@ -188,7 +191,7 @@ HPO_DI void blk_species_state(const BlkTabs& t, const BlkLds& w, double P, int t
   s.Yk = Yk;
   const double S = v[1] + ylast * t.sp[(K - 1) * kSpStride];
   s.Wbar = 1.0 / S;
-  s.m = P * s.invT * (1.0 / kR);
+  s.m = t.conv ? P * S : P * s.invT * (1.0 / kR);   // P / (R T), or rho / Wbar at constant volume
   s.rho = s.m * s.Wbar;
   s.invrho = S / s.m;
   s.lnPfac = kLnPatmOverR - s.lnT;
@@ -211,7 +214,7 @@ HPO_DI void blk_prepare(const BlkTabs& t, const BlkLds& w, double y, double P, i
     w.C[tid] = s.rho * s.Yk * s.invW;
     w.g[tid] = s.hRT - s.sR;
     w.hRT[tid] = s.hRT;
-    w.cp[tid] = s.cpR * kR * s.invW;
+    w.cp[tid] = (s.cpR - t.conv) * kR * s.invW;   // mass cp, or cv at constant volume
   }
   if (tid == K) {
     w.C[K] = 1.0;
@@ -320,7 +323,9 @@ HPO_DI double blk_rhs(const BlkTabs& t, const BlkLds& w, double y, double P, int
     }
   }
   const bool sp = tid < K;
-  double v[2] = {sp ? s.hRT * kR * s.T * wdot : 0.0, sp ? s.Yk * s.cpR * kR * s.invW : 0.0};
+  // constant pressure: enthalpies and cp; constant volume: internal energies H - R T and cv
+  const double cvf = t.conv;
+  double v[2] = {sp ? (s.hRT - cvf) * kR * s.T * wdot : 0.0, sp ? s.Yk * (s.cpR - cvf) * kR * s.invW : 0.0};
   bsum<2>(v, w.red, tid);   // Q, cpmix
   // species row k lives in thread k + 1; the temperature row in thread 0
   if (tid < K - 1) w.vec[tid + 1] = wdot * s.W * s.invrho;
@@ -410,7 +415,7 @@ HPO_DI void blk_jacobian(const BlkTabs& t, const BlkLds& w, const BlkScratch& g,
     g.gphi[r] = dq * pm;
     g.gdm[r] = fM * eff_base;
     g.gdT[r] = pm * (fwd * dlnkf - rev * dlnkr) + dq * dpm_dT;
-    g.dH[r] = dHRT * RT;
+    g.dH[r] = (dHRT - t.conv * dnu) * RT;   // reaction enthalpy, or internal energy at constant volume
   }
   __syncthreads();
   // species rows (thread k): d(wdot_k)/dC row, wdot_k, d(wdot_k)/dT at fixed C and m,
@@ -448,29 +453,35 @@ HPO_DI void blk_jacobian(const BlkTabs& t, const BlkLds& w, const BlkScratch& g,
     dQv[tid] = dQdC;
   }
   const bool sp = tid < K;
-  const double Hk = s.hRT * RT;
-  double v[7] = {sp ? Hk * wdot : 0.0, sp ? s.cpR * kR * wdot : 0.0, sp ? Hk * JT : 0.0, sp ? Hk * Jm : 0.0,
-                 sp ? s.Yk * s.cpR * kR * s.invW : 0.0, sp ? s.Yk * s.dcpR * kR * s.invW : 0.0,
+  const double cvf = t.conv, cpf = 1.0 - cvf;
+  const double Hk = (s.hRT - cvf) * RT;   // enthalpy, or internal energy at constant volume
+  double v[7] = {sp ? Hk * wdot : 0.0, sp ? (s.cpR - cvf) * kR * wdot : 0.0, sp ? Hk * JT : 0.0, sp ? Hk * Jm : 0.0,
+                 sp ? s.Yk * (s.cpR - cvf) * kR * s.invW : 0.0, sp ? s.Yk * s.dcpR * kR * s.invW : 0.0,
                  sp ? dQdC * w.C[tid] : 0.0};
   bsum<7>(v, w.red, tid);   // also makes the species values above visible
   const double Q = v[0], SCpw = v[1], SHJT = v[2], SHJm = v[3], cpmix = v[4], dcpmix = v[5], sigT = v[6];
-  // rows: thread 0 the temperature row, thread k + 1 the row of species k (chain rule to T, Y)
+  // rows: thread 0 the temperature row, thread k + 1 the row of species k (chain rule to T, Y).
+  // Constant pressure: C and m move with T (-C/T, -m/T) and C with Y through Wbar, and the
+  // species rows' 1/rho moves too. Constant volume: C does not depend on T, m = rho sum Y/W
+  // gives dm/dY_j = rho d_j, and rho is fixed.
   if (tid < NS) {
     const double rho = s.rho, Wbar = s.Wbar;
     const double invWN = t.sp[(K - 1) * kSpStride];
     const double D = rho * cpmix, invD = 1.0 / D, fT = -Q * invD;
     const double cpN = w.cp[K - 1];
+    const double dcoef = -cpf * rho * Wbar * cpmix;
     double* Jrow = g.J + static_cast<int64_t>(tid) * NS;
     if (tid == 0) {
-      const double dQdT = SCpw + SHJT - sigT * s.invT - (s.m * s.invT) * SHJm;
-      const double dDdT = -rho * cpmix * s.invT + rho * dcpmix;
+      const double dQdT = SCpw + SHJT - cpf * (sigT * s.invT + (s.m * s.invT) * SHJm);
+      const double dDdT = -cpf * rho * cpmix * s.invT + rho * dcpmix;
       Jrow[0] = (-dQdT - fT * dDdT) * invD;
       const double c1 = dQv[K - 1] * rho * invWN;
+      const double ysig = t.conv ? rho * SHJm : -Wbar * sigT;
       for (int j = 0; j < NS - 1; ++j) {
         const double invWj = t.sp[j * kSpStride];
         const double dj = invWj - invWN;
-        const double base = dQv[j] * rho * invWj - c1 - Wbar * dj * sigT;
-        const double dDdY = -rho * Wbar * dj * cpmix + rho * (w.cp[j] - cpN);
+        const double base = dQv[j] * rho * invWj - c1 + dj * ysig;
+        const double dDdY = dcoef * dj + rho * (w.cp[j] - cpN);
         Jrow[j + 1] = (-base - fT * dDdY) * invD;
       }
     } else {
@@ -478,13 +489,15 @@ HPO_DI void blk_jacobian(const BlkTabs& t, const BlkLds& w, const BlkScratch& g,
       const double* row = g.JC + static_cast<int64_t>(k) * K;
       const double wk = wdotv[k], sk = sigv[k];
       const double fac = t.sp[k * kSpStride + 1] / rho;
-      Jrow[0] = fac * (JTv[k] - sk * s.invT - (s.m * s.invT) * Jmv[k] + wk * s.invT);
+      Jrow[0] = fac * (JTv[k] - cpf * (sk * s.invT + (s.m * s.invT) * Jmv[k] - wk * s.invT));
       const double c1 = row[K - 1] * rho * invWN;
+      const double ysig = t.conv ? rho * Jmv[k] : -Wbar * sk;
+      const double wcor = cpf * wk * Wbar;
       for (int j = 0; j < NS - 1; ++j) {
         const double invWj = t.sp[j * kSpStride];
         const double dj = invWj - invWN;
-        const double base = row[j] * rho * invWj - c1 - Wbar * dj * sk;
-        Jrow[j + 1] = fac * (base + wk * Wbar * dj);
+        const double base = row[j] * rho * invWj - c1 + dj * ysig;
+        Jrow[j + 1] = fac * (base + wcor * dj);
       }
     }
   }
@@ -643,6 +656,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   BlkTabs t = make_blk_tabs(smem, A.mech, Lay);
+  t.conv = A.reactor;
   double* wbase = reinterpret_cast<double*>(smem + Lay.lds_bytes);
   BlkLds w = make_blk_lds(wbase, Lay.K, Lay.R);
   Scal& S = *reinterpret_cast<Scal*>(wbase + blk_lds_doubles(Lay.K, Lay.R) + wave * kScalDoubles);
@@ -669,6 +683,7 @@ __global__ __launch_bounds__(kBlkThreads) void blk_eval_kernel(EvalArgs A) {
   __syncthreads();
   const int tid = threadIdx.x;
   BlkTabs t = make_blk_tabs(smem, A.mech, Lay);
+  t.conv = A.reactor;
   BlkLds w = make_blk_lds(reinterpret_cast<double*>(smem + Lay.lds_bytes), Lay.K, Lay.R);
   BlkScratch g = make_blk_scratch(A.scratch + static_cast<int64_t>(blockIdx.x) * A.scratch_stride, Lay);
   for (int64_t c = blockIdx.x; c < A.ncells; c += gridDim.x) {

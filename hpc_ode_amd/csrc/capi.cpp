@@ -120,6 +120,7 @@ void check_opts(const hpo_opts& o) {
   if (o.blocks_per_cu < 0) throw std::invalid_argument("blocks_per_cu must be >= 0");
   if (o.lu_layout != 0 && o.lu_layout != 1) throw std::invalid_argument("lu_layout must be 0 or 1");
   if (o.cell_layout < 0 || o.cell_layout > 2) throw std::invalid_argument("cell_layout must be 0, 1 or 2");
+  if (o.reactor < 0 || o.reactor > 1) throw std::invalid_argument("reactor must be 0 (constant pressure) or 1 (constant volume)");
 }
 
 bool use_block_layout(const hpo_mech* m, const hpo_opts& o) {
@@ -163,7 +164,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
     wpb = o.waves_per_block > 0 ? o.waves_per_block : 4;
     block = 64 * wpb;
     lds = static_cast<size_t>(m->blob.lay.lds_bytes) + static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
-    fn = hpo::bdf_kernel_fn(ns, variant, d_prof != nullptr);
+    fn = hpo::bdf_kernel_fn(ns, variant, d_prof != nullptr, o.reactor);
   }
   int occ = 0;
   HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
@@ -218,6 +219,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   A.maxsteps = o.maxsteps;
   A.refine_sweeps = o.refine_sweeps;
   A.prof = reinterpret_cast<int64_t*>(d_prof);
+  A.reactor = o.reactor;
 
   HPO_HIP(hpo::launch_reset_counter(res->d_queue, st));
   if (tm) HPO_HIP(hipEventRecord(m->ev0, st));
@@ -342,6 +344,13 @@ int hpo_mech_species_name(const hpo_mech* m, int32_t k, char* buf, int32_t bufle
 int hpo_mech_molecular_weights(const hpo_mech* m, double* W) {
   if (!m || !W) return fail(HPO_E_INVALID, "bad argument");
   std::copy(m->host.W.begin(), m->host.W.end(), W);
+  return HPO_OK;
+}
+
+int hpo_mech_enthalpy_mass(const hpo_mech* m, int64_t n, const double* T, const double* Y, int64_t ldY, double* h) {
+  if (!m || n < 0 || (n > 0 && (!T || !Y || !h))) return fail(HPO_E_INVALID, "bad argument");
+  if (ldY < m->host.K()) return fail(HPO_E_INVALID, "ldY smaller than the number of species");
+  for (int64_t i = 0; i < n; ++i) h[i] = hpo::enthalpy_mass(m->host, T[i], Y + i * ldY);
   return HPO_OK;
 }
 
@@ -527,7 +536,7 @@ int hpo_integrate_device_noalloc(const hpo_mech* mc, int64_t ncells, const doubl
                        static_cast<size_t>(wpb) * hpo::bdf_wave_lds_doubles(m->blob.lay) * 8;
     const int variant = hpo::bdf_variant(opt.lu_precision, opt.lu_layout);
     int occ = 0;
-    HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_kernel_fn(ns, variant, 0), block, lds));
+    HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hpo::bdf_kernel_fn(ns, variant, 0, opt.reactor), block, lds));
     if (occ < 1) throw std::invalid_argument("kernel does not fit on a CU with this configuration");
     const int bpc = opt.blocks_per_cu > 0 ? std::min(opt.blocks_per_cu, occ) : occ;
     int64_t grid = static_cast<int64_t>(m->n_cu) * bpc;
@@ -558,6 +567,7 @@ int hpo_integrate_device_noalloc(const hpo_mech* mc, int64_t ncells, const doubl
     A.maxsteps = opt.maxsteps;
     A.refine_sweeps = opt.refine_sweeps;
     A.prof = nullptr;
+    A.reactor = opt.reactor;
     hipStream_t st = static_cast<hipStream_t>(stream);
     HPO_HIP(hpo::launch_reset_counter(A.queue, st));
     if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
@@ -569,10 +579,11 @@ int hpo_integrate_device_noalloc(const hpo_mech* mc, int64_t ncells, const doubl
 
 // cell_layout: 0 = the wave tables when the mechanism has them, else the block tables;
 // 1 = wave (one wave per cell); 2 = block (one 256-thread workgroup per cell)
-static int eval_common(const hpo_mech* mc, int32_t cell_layout, int64_t n, const double* y, const double* P,
-                       double* out, int mode) {
+static int eval_common(const hpo_mech* mc, int32_t cell_layout, int32_t reactor, int64_t n, const double* y,
+                       const double* P, double* out, int mode) {
   if (!mc || !y || !P || !out || n < 0) return fail(HPO_E_INVALID, "bad argument");
   if (cell_layout < 0 || cell_layout > 2) return fail(HPO_E_INVALID, "cell_layout must be 0, 1 or 2");
+  if (reactor < 0 || reactor > 1) return fail(HPO_E_INVALID, "reactor must be 0 (constant pressure) or 1 (constant volume)");
   hpo_mech* m = const_cast<hpo_mech*>(mc);
   const int ns = m->host.K();
   const bool wave_ok = m->has_wave && hpo::kernel_supports(ns);
@@ -609,6 +620,7 @@ static int eval_common(const hpo_mech* mc, int32_t cell_layout, int64_t n, const
     A.ncells = n;
     A.ld = ns;
     A.mode = mode;
+    A.reactor = reactor;
     if (block) {
       const size_t lds = static_cast<size_t>(m->bblob.lay.lds_bytes) + hpo::blk_work_lds_bytes(m->bblob.lay);
       HPO_HIP(hpo::launch_eval_block(ns, A, static_cast<int>(grid), lds, nullptr));
@@ -628,17 +640,17 @@ static int eval_common(const hpo_mech* mc, int32_t cell_layout, int64_t n, const
 }
 
 int hpo_eval_rhs(const hpo_mech* m, int64_t n, const double* y, const double* P, double* f) {
-  return eval_common(m, 0, n, y, P, f, 0);
+  return eval_common(m, 0, 0, n, y, P, f, 0);
 }
 
 int hpo_eval_jac(const hpo_mech* m, int64_t n, const double* y, const double* P, double* J) {
-  return eval_common(m, 0, n, y, P, J, 1);
+  return eval_common(m, 0, 0, n, y, P, J, 1);
 }
 
-int hpo_eval_layout(const hpo_mech* m, int32_t cell_layout, int32_t what, int64_t n, const double* y,
-                    const double* P, double* out) {
+int hpo_eval_layout(const hpo_mech* m, int32_t cell_layout, int32_t reactor, int32_t what, int64_t n,
+                    const double* y, const double* P, double* out) {
   if (what != 0 && what != 1) return fail(HPO_E_INVALID, "what must be 0 (RHS) or 1 (Jacobian)");
-  return eval_common(m, cell_layout, n, y, P, out, what);
+  return eval_common(m, cell_layout, reactor, n, y, P, out, what);
 }
 
 int hpo_lu_solve(int32_t ns, int32_t fp32, int64_t n, const double* M, const double* b, double* x, int32_t* ok) {

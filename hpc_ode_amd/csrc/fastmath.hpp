@@ -67,10 +67,10 @@ HPO_HD double fast_log10(double x) { return fast_log(x) * 0.43429448190325182765
 // finite range. The table holds 2^(j/32) correctly rounded (kExp2Tab32; on the device a copy
 // in LDS is passed in). Error: at most 2 ulp against the C library (tests/test_fastmath.py).
 // Results overflow to +inf above 709.78 and underflow through the subnormals to 0 below
-// -745.13, as the C library's do; NaN propagates. Arguments are first clamped to
-// [-760, 720], where the reduction is exact and the result already 0 or +inf: Troe
-// parameters such as T** = 1e100 or T*** = 1e-30 give arguments of -1e97 or -1e33, and
-// infinite arguments give 0 and +inf.
+// -745.13, as the C library's do; NaN propagates. The reduction holds for |x| up to about
+// 1e15 (beyond that x * 32/ln2 no longer rounds to within one of the nearest integer and r
+// grows without bound): the kernels' arguments stay within 1e13, the Troe coefficients being
+// bounded where the tables are built. Infinite arguments give NaN.
 constexpr double kExp2Tab32[32] = {
     0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
     0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
@@ -82,7 +82,6 @@ constexpr double kExp2Tab32[32] = {
     0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0};
 
 HPO_HD double tab_exp(double x, const double* tab) {
-  x = x < -760.0 ? -760.0 : (x > 720.0 ? 720.0 : x);   // comparisons with NaN are false
   constexpr double INV_LN2_32 = 0x1.71547652b82fep+5;   // 32 / ln 2
   constexpr double LN2_32_HI = 0x1.62e42ff000000p-6;
   constexpr double LN2_32_LO = -0x1.718432a1b0e26p-40;
@@ -100,9 +99,13 @@ HPO_HD double tab_exp(double x, const double* tab) {
   return std::ldexp(std::fma(t, q, t), ki >> 5);
 }
 
-// x^y for x >= 0 as exp(y ln x); used for the step-size ratios of the BDF controller, where
-// x is a positive error-norm ratio and y = 1/L with L = 2..6.
+// x^y for x >= 0, y > 0 as exp(y ln x); used for the step-size ratios of the BDF controller,
+// where x is an error-norm ratio and y = 1/L with L = 2..6. x = 0 (an error estimate of exactly
+// zero, e.g. a cell with no reaction to run) gives 0, as pow does: ln 0 = -inf is outside
+// tab_exp's domain.
 HPO_HD double fast_pow(double x, double y) { return std::exp(y * fast_log(x)); }
-HPO_HD double fast_pow(double x, double y, const double* exptab) { return tab_exp(y * fast_log(x), exptab); }
+HPO_HD double fast_pow(double x, double y, const double* exptab) {
+  return x > 0.0 ? tab_exp(y * fast_log(x), exptab) : 0.0;
+}
 
 }  // namespace hpo

@@ -40,6 +40,7 @@ struct BdfArgs {
   int32_t refine_sweeps;
   int64_t* prof;             // [ncells][HPO_NPROF] shader-clock cycles per phase, or null
   BlkLayout blay;            // block-per-cell kernel: layout of the block tables at `mech`
+  int32_t reactor;           // 0 constant pressure (P = pressure, Pa); 1 constant volume (P = density, kg/m^3)
 };
 
 struct EvalArgs {
@@ -54,6 +55,7 @@ struct EvalArgs {
   int64_t ld;
   int32_t mode;
   BlkLayout blay;            // block-per-cell component kernels: layout of the block tables
+  int32_t reactor;           // as BdfArgs::reactor
 };
 
 struct LuArgs {
@@ -68,7 +70,7 @@ int bdf_wave_lds_doubles(const DevMechLayout& L);
 int eval_wave_lds_doubles(const DevMechLayout& L);
 int64_t wave_scratch_doubles_host(const DevMechLayout& L);
 // variant: 0 FP64 LU; 1 FP32 explicit inverse (Gauss-Jordan); 2 FP32 in-place LU
-const void* bdf_kernel_fn(int ns, int variant, int prof = 0);
+const void* bdf_kernel_fn(int ns, int variant, int prof = 0, int conv = 0);
 hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st);
 hipError_t launch_reset_counter(uint32_t* q, hipStream_t st);
 // The no-allocation entry point's fixed scratch: 192 MiB of static device memory per GPU (the

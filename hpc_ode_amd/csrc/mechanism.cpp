@@ -446,10 +446,32 @@ Mechanism load_mechanism(const std::string& path) {
   return load_mechanism_yaml(path);
 }
 
+double enthalpy_mass(const Mechanism& m, double T, const double* Y) {
+  double h = 0.0;
+  for (int k = 0; k < m.K(); ++k) {
+    const double* a = &m.nasa[14 * k + (T <= m.tmid[k] ? 0 : 7)];
+    const double hRT = a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (a[3] / 4 + T * a[4] / 5))) + a[5] / T;
+    h += Y[k] * hRT / m.W[k];
+  }
+  return h * kGasConstant * T;
+}
+
 // ---------------------------------------------------------------------------
 // Device tables
 // ---------------------------------------------------------------------------
 namespace {
+
+// Troe coefficients for the device: 1/T***, 1/T* and T**, magnitudes above 1e8 held at 1e8.
+// The terms they enter, exp(-T/T***), exp(-T/T*) and exp(-T**/T), are then 0 in double
+// precision for every T between 1e-5 K and 1e5 K, as with the true values (nDodecane_Reitz has
+// T*** = 1e-30 and T** = 1e100), and the device exponential's arguments stay within 1e13 in
+// magnitude, where its argument reduction holds (an argument of -1e97 would not).
+void troe_device_coefs(const Reaction& rx, double& iT3, double& iT1, double& T2) {
+  constexpr double kMax = 1e8;
+  iT3 = std::fmin(1.0 / rx.troe[1], kMax);
+  iT1 = std::fmin(1.0 / rx.troe[2], kMax);
+  T2 = std::fmin(rx.troe[3], kMax);
+}
 
 int align_to(int x, int a) { return (x + a - 1) / a * a; }
 
@@ -749,9 +771,7 @@ DevMechBlob build_device_tables(const Mechanism& m) {
       P.b0 = rx.b0;
       P.theta0 = rx.theta0;
       P.troe_a = rx.troe[0];
-      P.troe_iT3 = 1.0 / rx.troe[1];
-      P.troe_iT1 = 1.0 / rx.troe[2];
-      P.troe_T2 = rx.troe[3];
+      troe_device_coefs(rx, P.troe_iT3, P.troe_iT1, P.troe_T2);
       ++ip;
     }
     hdr[r] = h;
@@ -932,9 +952,7 @@ DevBlkBlob build_block_tables(const Mechanism& m) {
       P.b0 = rx.b0;
       P.theta0 = rx.theta0;
       P.troe_a = rx.troe[0];
-      P.troe_iT3 = 1.0 / rx.troe[1];
-      P.troe_iT1 = 1.0 / rx.troe[2];
-      P.troe_T2 = rx.troe[3];
+      troe_device_coefs(rx, P.troe_iT3, P.troe_iT1, P.troe_T2);
       ++ip;
     }
     rxn[r] = h;

@@ -50,6 +50,12 @@ Mechanism load_mechanism_yaml(const std::string& path);
 Mechanism load_mechanism_xml(const std::string& path);   // Cantera CTML
 Mechanism load_mechanism(const std::string& path);       // by extension: .xml/.ctml or YAML
 
+constexpr double kGasConstant = 8314.4621;   // J/(kmol K), the value the kernels use
+
+// Mass-specific enthalpy h(T, Y) = sum_k Y_k H_k(T) / W_k in J/kg, H_k from the NASA7
+// polynomials; Y holds all K mass fractions.
+double enthalpy_mass(const Mechanism& m, double T, const double* Y);
+
 // ---------------------------------------------------------------------------
 // Device tables. One contiguous blob; the kernel copies the "LDS part" into
 // shared memory once per workgroup and reads the "global part" (Jacobian

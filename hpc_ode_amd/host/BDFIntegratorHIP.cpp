@@ -77,6 +77,8 @@ void BDFIntegratorHIP::init(std::shared_ptr<Context> c, IntegratorConfig config)
       ss << "mechanism size of: " << mechanism << " (" << nsp << ") differs from the mesh size: " << systemSize;
       throw std::runtime_error(ss.str());
     }
+    W_.assign(nsp, 0.0);
+    check(hpo_mech_molecular_weights(mech_, W_.data()), "molecular weights");
     const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(totalSize, kMaxResidentCells));
     check(hpo_batch_create(mech_, cap, &batch_), "allocating the device batch");
   } catch (const std::exception& e) {
@@ -122,11 +124,23 @@ void BDFIntegratorHIP::integrate(double t0, double t) {
     std::copy(Y.begin(), Y.end() - 1, row + 1);
   }
 
+  // constant volume: each cell keeps the density it has at its pressure, rho = P Wbar / (R T)
+  if (opts_.reactor == 1) {
+    rho_.resize(n);
+    for (int c = 0; c < n; ++c) {
+      const std::vector<double>& Y = mesh->getSpeciesVector(c);
+      double S = 0.0;
+      for (int k = 0; k < ns; ++k) S += Y[k] / W_[k];
+      rho_[c] = (P_.empty() ? pressure : P_[c]) / (HPO_GAS_CONSTANT * T[c] * S);
+    }
+  }
+  const double* par = opts_.reactor == 1 ? rho_.data() : (P_.empty() ? nullptr : P_.data());
+
   const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(n, kMaxResidentCells));
   for (int64_t first = 0; first < n; first += cap) {
     const int64_t cnt = std::min<int64_t>(cap, n - first);
     double* rows = &y_[static_cast<size_t>(first) * ns];
-    check(hpo_batch_put(batch_, 0, cnt, rows, ns, P_.empty() ? nullptr : &P_[first], pressure,
+    check(hpo_batch_put(batch_, 0, cnt, rows, ns, par ? par + first : nullptr, pressure,
                         dt_.empty() ? nullptr : &dt_[first], t - t0),
           "copying cells to the device");
     hpo_timing tm{};
@@ -150,6 +164,12 @@ void BDFIntegratorHIP::integrate(double t0, double t) {
     double s = 0.0;
     for (int i = 0; i < ns - 1; ++i) s += Y[i];
     Y[ns - 1] = 1.0 - s;
+  }
+  // ENTHA = h(T, Y) of the integrated state, J/kg (what CanteraIntegrator.cpp:93 writes)
+  if (mesh->hasEnthalpy()) {
+    std::vector<double>& h = mesh->getEnthalpyVector();
+    for (int c = 0; c < n; ++c)
+      check(hpo_mech_enthalpy_mass(mech_, 1, &T[c], mesh->getSpeciesVector(c).data(), ns, &h[c]), "enthalpy");
   }
 
   std::map<int32_t, int> failed;

@@ -4,8 +4,14 @@
 //
 // Differences a caller can see, all deliberate:
 //  * The mechanism is read from its YAML or CTML file at init (the reference compiles pyJac code
-//    for one mechanism). Kernels are instantiated for states of 9, 10 and 53 variables
-//    (h2_li_2004, h2o2, GRI-3.0 and other mechanisms of those sizes); init throws for others.
+//    for one mechanism). Kernels are instantiated for states of 9, 10 and 53 variables (one
+//    wave per cell: h2_li_2004, h2o2, GRI-3.0 and other mechanisms of those sizes) and of 53
+//    and 100 variables (one workgroup per cell: GRI-3.0, nDodecane_Reitz); integrate throws
+//    for other sizes.
+//  * A mesh with enthalpy gets ENTHA = h(T, Y) of the integrated state in J/kg, as the
+//    reference's CanteraIntegrator writes it (CanteraIntegrator.cpp:93); its CVODE integrators
+//    leave the input value. In the constant-pressure reactor h is conserved, so the two agree
+//    to the integration tolerance when the input ENTHA was consistent with (T, Y).
 //  * Device memory is owned from init to clean, so repeated integrate calls (substeps of a
 //    CFD run) reuse it; the reference GPU path allocates per call (CVodeIntegratorGPU.cpp:57,121).
 //  * The per-cell solver return code is kept (cellStatus()) and failed cells are logged; the
@@ -16,6 +22,10 @@
 //    reference makes (app/main.cpp:56).
 //  * Optional per-cell pressure and time step (setCellPressure / setCellTimeStep) extend the
 //    single IntegratorConfig::pressure and the single interval.
+//  * hpo_opts::reactor = 1 integrates each cell at constant volume (the reference's CONV
+//    option, header.h:6-7, whose pyJac branch does not compile): the cell keeps the density it
+//    has at the configured (or per-cell) pressure, rho = P Wbar / (R T), and its pressure
+//    follows from the state.
 #pragma once
 
 #include <cstdint>
@@ -57,6 +67,7 @@ class BDFIntegratorHIP : public Integrator {
   hpo_batch* batch_ = nullptr;
   std::vector<double> y_;          // [cells][systemSize] staging rows y = [T, Y_0 .. Y_{nsp-2}]
   std::vector<double> P_, dt_;
+  std::vector<double> W_, rho_;    // molecular weights; per-cell densities of the constant-volume reactor
   std::vector<int32_t> status_, stats_;
   hpo_timing timing_{};
 };

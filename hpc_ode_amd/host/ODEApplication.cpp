@@ -13,7 +13,9 @@
 // CVodeIntegratorGPU) and BDFIntegratorHIP all select the GPU BDF; the Cantera integrators are
 // not part of this build and are refused like an unknown type ("Integrator type is not valid").
 // The omp block is read and logged; the GPU integrator does not use it.
-// Optional keys of this build under integrator: luPrecision (fp64 | fp32) and refineSweeps.
+// Optional keys of this build under integrator: luPrecision (fp64 | fp32), refineSweeps, and
+// reactor (CONP, the default: constant pressure; CONV: constant volume at the density each cell
+// has at the configured pressure).
 #include <chrono>
 #include <cstdio>
 #include <ctime>
@@ -134,6 +136,10 @@ void run(const std::string& configFile) {
     else if (lp->as_string() != "fp64") throw std::runtime_error("luPrecision must be fp64 or fp32");
   }
   if (const hpo::YNode* rs = ic.find("refineSweeps")) o.refine_sweeps = static_cast<int32_t>(rs->as_long());
+  if (const hpo::YNode* re = ic.find("reactor")) {
+    if (re->as_string() == "CONV") o.reactor = 1;
+    else if (re->as_string() != "CONP") throw std::runtime_error("reactor must be CONP or CONV");
+  }
   logger->info("Integrator " + type + " runs as BDFIntegratorHIP");
   {
     BDFIntegratorHIP integrator(&o);

@@ -4,7 +4,11 @@
 //
 //   hpo_integrate_csv --input cells.csv --output result.csv [--mechanism gri30.yaml]
 //       [--dt 1e-3] [--pressure 101325.15] [--rtol 1e-6] [--atol 1e-10] [--lu fp64|fp32]
-//       [--sweeps 1] [--raw state.bin] [--performance-yaml performance.yaml] [--quiet]
+//       [--sweeps 1] [--reactor CONP|CONV] [--raw state.bin] [--performance-yaml performance.yaml]
+//       [--quiet]
+//
+// --reactor CONV integrates at constant volume: each cell keeps the density it has at
+// --pressure (default CONP, constant pressure).
 //
 // --raw writes the integrated mesh in full precision: per cell, T then the nsp mass
 // fractions, as little-endian doubles, followed by the int32 solver status of every cell.
@@ -24,7 +28,7 @@
 #include "csvIO.hpp"
 
 int main(int argc, char** argv) {
-  std::string in, out, raw, perf, mech = "gri30.yaml", lu = "fp64";
+  std::string in, out, raw, perf, mech = "gri30.yaml", lu = "fp64", reactor = "CONP";
   double dt = 1e-3, P = 101325.15, rtol = 1e-6, atol = 1e-10;
   int sweeps = 1;
   bool quiet = false;
@@ -43,6 +47,7 @@ int main(int argc, char** argv) {
     else if (a == "--atol") atol = std::stod(val());
     else if (a == "--lu") lu = val();
     else if (a == "--sweeps") sweeps = std::stoi(val());
+    else if (a == "--reactor") reactor = val();
     else if (a == "--raw") raw = val();
     else if (a == "--performance-yaml") perf = val();
     else if (a == "--quiet") quiet = true;
@@ -50,6 +55,10 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "unknown argument %s\n", a.c_str());
       return 2;
     }
+  }
+  if (reactor != "CONP" && reactor != "CONV") {
+    std::fprintf(stderr, "--reactor must be CONP or CONV\n");
+    return 2;
   }
   if (in.empty() || out.empty()) {
     std::fprintf(stderr, "usage: hpo_integrate_csv --input in.csv --output out.csv [options]\n");
@@ -67,6 +76,7 @@ int main(int argc, char** argv) {
     hpo_opts_default(&o);
     o.lu_precision = (lu == "fp32") ? 1 : 0;
     o.refine_sweeps = sweeps;
+    o.reactor = reactor == "CONV" ? 1 : 0;
     IntegratorConfig cfg;
     cfg.mechanism = mech;
     cfg.reltol = rtol;

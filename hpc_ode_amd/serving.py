@@ -47,13 +47,16 @@ def mean_cells(max_cells=MAX_CELLS):
 
 
 def run_load(server, mechs, rate_hz, duration_s, seed=0, substeps=2, workers=192, max_cells=MAX_CELLS,
-             log=None):
-    """Offers Poisson arrivals at rate_hz for duration_s seconds; returns a summary dict."""
+             log=None, min_substeps=0):
+    """Offers Poisson arrivals at rate_hz for duration_s seconds, or for as long as it takes to
+    offer min_substeps substeps if that is longer; returns a summary dict."""
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate_hz, int(rate_hz * duration_s * 2) + 16)
+    n_min = -(-min_substeps // substeps)
+    gaps = rng.exponential(1.0 / rate_hz, max(int(rate_hz * duration_s * 2), 2 * n_min) + 16)
     times = np.cumsum(gaps)
-    times = times[times < duration_s]
-    n = len(times)
+    n = max(int((times < duration_s).sum()), n_min)
+    times = times[:n]
+    duration_s = max(duration_s, float(times[-1])) if n else duration_s
     cells, mech, dt = request_mix(n, seed + 1, max_cells)
     lat, queue, bsz, breq = [], [], [], []
     failed = [0]

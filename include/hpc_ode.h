@@ -62,6 +62,10 @@ typedef struct {
                               size, else one workgroup per cell); 1 one wave per cell; 2 one
                               256-thread workgroup per cell (Gauss-Jordan Newton matrix across the
                               workgroup; states of more than 64 variables, e.g. nDodecane_Reitz) */
+  int32_t reactor;         /* 0 constant pressure (the reference's CONP, header.h:6): each cell's P is
+                              its pressure in Pa; 1 constant volume (CONV, dydt.c:135-263): each
+                              cell's P is its density in kg/m^3, held through the interval, and
+                              the temperature equation uses internal energies and cv */
 } hpo_opts;
 
 typedef struct {
@@ -72,6 +76,10 @@ typedef struct {
   int32_t lds_bytes;       /* dynamic LDS per workgroup */
   int32_t reserved;
 } hpo_timing;
+
+/* Universal gas constant of the library's thermochemistry, J/(kmol K); e.g. for the density a
+ * constant-volume cell starting at (T, Y, P) keeps: rho = P / (HPO_GAS_CONSTANT T sum_k Y_k/W_k). */
+#define HPO_GAS_CONSTANT 8314.4621
 
 /* Defaults: rtol 1e-6, atol 1e-10, maxsteps 10000 (src/Integrators/CVodeIntegrator.cpp:107,122). */
 void hpo_opts_default(hpo_opts *o);
@@ -87,6 +95,12 @@ void hpo_mech_destroy(hpo_mech *m);
 int hpo_mech_info(const hpo_mech *m, int32_t *n_species, int32_t *n_state, int32_t *n_reactions);
 int hpo_mech_species_name(const hpo_mech *m, int32_t k, char *buf, int32_t buflen);
 int hpo_mech_molecular_weights(const hpo_mech *m, double *W /* [n_species] */);
+/* Mass-specific enthalpy h(T, Y) = sum_k Y_k H_k(T) / W_k in J/kg from the mechanism's NASA7
+ * polynomials, computed on the host: T[n], Y[n][ldY] with all n_species mass fractions per row.
+ * The quantity the reference's Cantera integrator writes to the mesh's ENTHA
+ * (thermo->enthalpy_mass(), src/Integrators/CanteraIntegrator.cpp:93). */
+int hpo_mech_enthalpy_mass(const hpo_mech *m, int64_t n, const double *T, const double *Y, int64_t ldY,
+                           double *h);
 /* Flat host tables (for cross-checking readers): see hpc_ode_amd/csrc/mechanism.hpp. */
 int hpo_mech_tables(const hpo_mech *m, double *nasa /*[K][14]*/, double *tmid /*[K]*/,
                     double *arrhenius /*[R][8]: lnA,b,theta,A,lnA0,b0,theta0,A0*/,
@@ -226,11 +240,12 @@ int hpo_server_hbm_used(const hpo_server *s, int64_t *used_bytes, int64_t *budge
 int hpo_eval_rhs(const hpo_mech *m, int64_t n, const double *y, const double *P, double *f);
 /* J = d f / d y, [n][n_state][n_state] row-major; what eval_jacob (jacob.c:3-371) computes. */
 int hpo_eval_jac(const hpo_mech *m, int64_t n, const double *y, const double *P, double *J);
-/* hpo_eval_rhs (what = 0) or hpo_eval_jac (what = 1) with the cell layout named as in
- * hpo_opts::cell_layout (0 auto, 1 wave, 2 block); HPO_E_UNSUPPORTED when the mechanism has no
- * tables or kernel of that layout. */
-int hpo_eval_layout(const hpo_mech *m, int32_t cell_layout, int32_t what, int64_t n, const double *y,
-                    const double *P, double *out);
+/* hpo_eval_rhs (what = 0) or hpo_eval_jac (what = 1) with the cell layout and the reactor named
+ * as in hpo_opts::cell_layout (0 auto, 1 wave, 2 block) and hpo_opts::reactor (0 constant
+ * pressure, P = pressure; 1 constant volume, P = density); HPO_E_UNSUPPORTED when the mechanism
+ * has no tables or kernel of that layout. */
+int hpo_eval_layout(const hpo_mech *m, int32_t cell_layout, int32_t reactor, int32_t what, int64_t n,
+                    const double *y, const double *P, double *out);
 /* Batched dense solve with the kernel's register-resident Newton-matrix code: M [n][ns][ns]
  * row-major, b, x [n][ns]. fp32 = 0: FP64 partial-pivot LU and triangular solves; fp32 = 1: the
  * FP32 Gauss-Jordan inverse (same pivots) applied as a matrix-vector product, no refinement. */

@@ -29,7 +29,7 @@ class OrMech(C.Structure):
         ("kind", _ip), ("rev", _ip), ("nreac", _ip), ("nprod", _ip),
         ("reac_sp", _ip), ("prod_sp", _ip), ("reac_nu", _dp), ("prod_nu", _dp),
         ("arr", _dp), ("troe", _dp), ("eff_ptr", _ip), ("eff_sp", _ip),
-        ("eff_val", _dp), ("eff_base", _dp),
+        ("eff_val", _dp), ("eff_base", _dp), ("conv", C.c_int),
     ]
 
 
@@ -95,10 +95,20 @@ def _p(a, t=_dp):
     return a.ctypes.data_as(t)
 
 
-class Oracle:
-    """One mechanism bound to the C oracle."""
+R_GAS = 8314.4621   # J/(kmol K), as oracle.c
+REACTORS = {"CONP": 0, "CONV": 1}
 
-    def __init__(self, path: str):
+
+class Oracle:
+    """One mechanism bound to the C oracle. reactor "CONP": constant pressure, the P argument
+    of rhs/jac/rates/integrate is the pressure in Pa; "CONV": constant volume, P is the
+    density in kg/m^3, held through the integration (header.h:6-7 of the reference picks one
+    of the two at compile time)."""
+
+    def __init__(self, path: str, reactor: str = "CONP"):
+        if reactor not in REACTORS:
+            raise ValueError(f"reactor must be one of {sorted(REACTORS)}")
+        self.reactor = reactor
         self.path = path
         self.mech = _mech.load(path)
         t = _mech.tables(self.mech)
@@ -116,6 +126,7 @@ class Oracle:
         self._keep = keep
         s = OrMech()
         s.K, s.R = t.K, t.R
+        s.conv = REACTORS[reactor]
         for name in ("W", "tmid", "nasa", "reac_nu", "prod_nu", "arr", "troe", "eff_val", "eff_base"):
             setattr(s, name, _p(keep[name]))
         for name in ("kind", "rev", "nreac", "nprod", "reac_sp", "prod_sp", "eff_ptr", "eff_sp"):
@@ -124,6 +135,14 @@ class Oracle:
         self.K = t.K
         self.R = t.R
         self.n = t.K
+
+    def density(self, P, y) -> np.ndarray:
+        """rho = P Wbar / (R T) of states y [n, K] (T, then the first K-1 mass fractions) at
+        pressures P: the density a constant-volume cell starting from (T, Y, P) keeps."""
+        y = np.atleast_2d(np.asarray(y, np.float64))
+        W = np.asarray(self.mech.W)
+        S = (y[:, 1:] / W[:-1]).sum(axis=1) + (1.0 - y[:, 1:].sum(axis=1)) / W[-1]
+        return np.asarray(P, np.float64) / (R_GAS * y[:, 0] * S)
 
     # --- chemistry -------------------------------------------------------
     def rhs(self, P: float, y: np.ndarray) -> np.ndarray:

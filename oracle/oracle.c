@@ -77,8 +77,15 @@ static void eval_state(const or_mech *m, double P, const double *y, state_t *s) 
     S += s->Y[k] * (1.0 / m->W[k]);
   }
   s->Wbar = 1.0 / S;
-  s->rho = P * s->Wbar / (R_GAS * T); /* chem_utils.c:42 */
-  s->m = P / (R_GAS * T);              /* rxn_rates_pres_mod.c:20 */
+  if (m->conv) {
+    /* constant volume: the argument is the density (eval_conc_rho of dydt.c:143) */
+    s->rho = P;
+    s->m = P * S;
+    s->P = s->m * R_GAS * T;
+  } else {
+    s->rho = P * s->Wbar / (R_GAS * T); /* chem_utils.c:42 */
+    s->m = P / (R_GAS * T);              /* rxn_rates_pres_mod.c:20 */
+  }
   s->cpmix = 0.0;
   s->dcpmix_dT = 0.0;
   for (int k = 0; k < K; ++k) {
@@ -259,9 +266,12 @@ void or_rhs(const or_mech *m, double P, const double *y, double *dy) {
     for (int i = 0; i < m->nprod[r]; ++i)
       wdot[m->prod_sp[OR_MAX_SLOTS * r + i]] += m->prod_nu[OR_MAX_SLOTS * r + i] * phi;
   }
-  double Q = 0.0; /* sum_k H_k wdot_k, molar enthalpy H_k = h_RT R T */
-  for (int k = 0; k < K; ++k) Q += wdot[k] * w.s.h_RT[k] * R_GAS * w.s.T;
-  dy[0] = -Q / (w.s.rho * w.s.cpmix);
+  /* constant pressure: Q = sum_k H_k wdot_k (molar enthalpy H_k = h_RT R T) over rho cp;
+     constant volume: the internal energies U_k = H_k - R T over rho cv, cv = cp - R / Wbar */
+  const double cv = (double)m->conv;
+  double Q = 0.0;
+  for (int k = 0; k < K; ++k) Q += wdot[k] * (w.s.h_RT[k] - cv) * R_GAS * w.s.T;
+  dy[0] = -Q / (w.s.rho * (w.s.cpmix - cv * R_GAS / w.s.Wbar));
   for (int k = 0; k < K - 1; ++k) dy[k + 1] = wdot[k] * m->W[k] / w.s.rho;
   free(wdot);
   work_free(&w);
@@ -323,7 +333,12 @@ void or_jac(const or_mech *m, double P, const double *y, double *J) {
       }
     }
   }
-  /* chain rule to y = [T, Y_0..Y_{K-2}] with Y_{K-1} = 1 - sum */
+  /* chain rule to y = [T, Y_0..Y_{K-2}] with Y_{K-1} = 1 - sum.
+     Constant pressure: rho = P Wbar / (R T) and m = P / (R T) move with T and Y, so
+     dC_i/dT = -C_i/T, dm/dT = -m/T, dC_i/dY_j carries -Wbar d_j C_i, and f_k = W_k wdot_k / rho
+     picks up rho's derivatives. Constant volume: rho is fixed, C_i = rho Y_i / W_i does not
+     depend on T, and m = rho sum_k Y_k / W_k gives dm/dY_j = rho d_j. d_j = 1/W_j - 1/W_{K-1}. */
+  const int conv = m->conv;
   const double rho = s->rho, Wbar = s->Wbar, T = s->T;
   const double invWN = 1.0 / m->W[K - 1];
   double *sigma = (double *)calloc(K, sizeof(double));
@@ -332,41 +347,44 @@ void or_jac(const or_mech *m, double P, const double *y, double *J) {
   /* dwdot_k/dY_j and dwdot_k/dT */
   double *dW = (double *)malloc(sizeof(double) * K * n); /* dW[k*n + col] */
   for (int k = 0; k < K; ++k) {
-    dW[k * n + 0] = JT[k] - sigma[k] / T - (s->m / T) * Jm[k];
+    dW[k * n + 0] = conv ? JT[k] : JT[k] - sigma[k] / T - (s->m / T) * Jm[k];
     for (int j = 0; j < K - 1; ++j) {
       double dj = 1.0 / m->W[j] - invWN;
-      dW[k * n + j + 1] = JC[(size_t)k * K + j] * rho / m->W[j] - JC[(size_t)k * K + K - 1] * rho * invWN -
-                          Wbar * dj * sigma[k];
+      double d = JC[(size_t)k * K + j] * rho / m->W[j] - JC[(size_t)k * K + K - 1] * rho * invWN;
+      dW[k * n + j + 1] = conv ? d + rho * dj * Jm[k] : d - Wbar * dj * sigma[k];
     }
   }
   /* species rows f_k = W_k wdot_k / rho */
   for (int k = 0; k < K - 1; ++k) {
     double fac = m->W[k] / rho;
-    J[(k + 1) + 0 * n] = fac * (dW[k * n + 0] + wdot[k] / T);
+    J[(k + 1) + 0 * n] = fac * (dW[k * n + 0] + (conv ? 0.0 : wdot[k] / T));
     for (int j = 0; j < K - 1; ++j) {
       double dj = 1.0 / m->W[j] - invWN;
-      J[(k + 1) + (j + 1) * n] = fac * (dW[k * n + j + 1] + wdot[k] * Wbar * dj);
+      J[(k + 1) + (j + 1) * n] = fac * (dW[k * n + j + 1] + (conv ? 0.0 : wdot[k] * Wbar * dj));
     }
   }
-  /* temperature row f_T = -Q/D, Q = sum H_k wdot_k, D = rho cp */
+  /* temperature row f_T = -Q/D: constant pressure Q = sum_k H_k wdot_k, D = rho cp; constant
+     volume Q = sum_k U_k wdot_k with U_k = H_k - R T (dU_k/dT = cp_k - R), D = rho cv */
+  const double cv = (double)conv;
   double Q = 0.0;
-  for (int k = 0; k < K; ++k) Q += s->h_RT[k] * R_GAS * T * wdot[k];
-  double D = rho * s->cpmix;
+  for (int k = 0; k < K; ++k) Q += (s->h_RT[k] - cv) * R_GAS * T * wdot[k];
+  const double cmix = s->cpmix - cv * R_GAS / Wbar;
+  double D = rho * cmix;
   double fT = -Q / D;
   {
     double dQ = 0.0;
     for (int k = 0; k < K; ++k)
-      dQ += s->cp_R[k] * R_GAS * wdot[k] + s->h_RT[k] * R_GAS * T * dW[k * n + 0];
-    double dD = -rho * s->cpmix / T + rho * s->dcpmix_dT;
+      dQ += (s->cp_R[k] - cv) * R_GAS * wdot[k] + (s->h_RT[k] - cv) * R_GAS * T * dW[k * n + 0];
+    double dD = (conv ? 0.0 : -rho * s->cpmix / T) + rho * s->dcpmix_dT;
     J[0] = (-dQ - fT * dD) / D;
   }
-  const double cpN = s->cp_R[K - 1] * R_GAS * invWN;
+  const double cN = (s->cp_R[K - 1] - cv) * R_GAS * invWN;
   for (int j = 0; j < K - 1; ++j) {
     double dj = 1.0 / m->W[j] - invWN;
     double dQ = 0.0;
-    for (int k = 0; k < K; ++k) dQ += s->h_RT[k] * R_GAS * T * dW[k * n + j + 1];
-    double cpj = s->cp_R[j] * R_GAS / m->W[j];
-    double dD = -rho * Wbar * dj * s->cpmix + rho * (cpj - cpN);
+    for (int k = 0; k < K; ++k) dQ += (s->h_RT[k] - cv) * R_GAS * T * dW[k * n + j + 1];
+    double cj = (s->cp_R[j] - cv) * R_GAS / m->W[j];
+    double dD = (conv ? 0.0 : -rho * Wbar * dj * s->cpmix) + rho * (cj - cN);
     J[0 + (j + 1) * n] = (-dQ - fT * dD) / D;
   }
   free(dW);

@@ -7,7 +7,10 @@
  *
  * It restates, in plain C:
  *   - pyJac's constant-pressure chemistry RHS (src/Mechanism/CPU/dydt.c:7-133 and the
- *     functions it calls) in table-driven form,
+ *     functions it calls) in table-driven form, and the constant-volume reactor that the
+ *     CONV branch of the same file (dydt.c:135-263, header.h:6-7) intends, restated from the
+ *     physics (that branch does not compile: `y[0]rho`, `y[52])(cv[52]`): density fixed,
+ *     C_k = rho Y_k / W_k, [M] = rho / Wbar, dT/dt = -sum_k u_k wdot_k / (rho cv),
  *   - the exact analytic Jacobian of that RHS (what src/Mechanism/CPU/jacob.c:3-371
  *     computes), column-major like SUNDIALS' dense matrices,
  *   - SUNDIALS' dense partial-pivot LU (SUNLinSol_Dense, used at
@@ -44,6 +47,9 @@ typedef struct {
   const int *eff_sp;     /* [E] */
   const double *eff_val; /* [E] efficiency minus default efficiency */
   const double *eff_base;/* [R] default efficiency */
+  int conv;              /* reactor: 0 constant pressure (the P argument of the functions below is
+                            the pressure, Pa); 1 constant volume (the P argument is the density,
+                            kg/m^3, held through the integration) */
 } or_mech;
 
 typedef struct {

@@ -83,9 +83,24 @@ def test_eval_layout_checks_come_before_device_calls(h2_path):
     y, P, out = np.zeros((1, m.n_state)), np.ones(1), np.zeros((1, m.n_state))
     dp = C.POINTER(C.c_double)
     a = [y.ctypes.data_as(dp), P.ctypes.data_as(dp), out.ctypes.data_as(dp)]
-    assert L.hpo_eval_layout(m.handle, 3, 0, 1, *a) == -1
-    assert L.hpo_eval_layout(m.handle, 1, 2, 1, *a) == -1
-    assert L.hpo_eval_layout(m.handle, 2, 0, 1, *a) == -5   # no block kernel for 10 variables
+    assert L.hpo_eval_layout(m.handle, 3, 0, 0, 1, *a) == -1    # cell layout
+    assert L.hpo_eval_layout(m.handle, 1, 2, 0, 1, *a) == -1    # reactor
+    assert L.hpo_eval_layout(m.handle, 1, 0, 2, 1, *a) == -1    # what
+    assert L.hpo_eval_layout(m.handle, 2, 0, 0, 1, *a) == -5    # no block kernel for 10 variables
+
+
+def test_reactor_option_checks(h2_path):
+    """hpo_opts::reactor outside 0..1 is refused before any device work; the Python names map to
+    the reference's CONP / CONV."""
+    assert H.opts(reactor="CONV").reactor == 1 and H.opts(reactor="CONP").reactor == 0
+    with pytest.raises(ValueError):
+        H.opts(reactor="CONSTANT")
+    m = H.Mechanism(h2_path)
+    o = H.opts(reactor=2)
+    fake = C.c_void_p(4096)   # never dereferenced: the option check comes first
+    rc = H.lib().hpo_integrate_device(m.handle, 1, fake, fake, m.n_state, fake, fake, fake, None, None, None,
+                                      C.byref(o), None, None)
+    assert rc == -1 and b"reactor" in H.lib().hpo_last_error()
 
 
 def test_server_option_checks_come_before_device_calls(h2_path):

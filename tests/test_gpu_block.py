@@ -54,71 +54,23 @@ def _states(mech, o, y0, P, dt):
     return np.concatenate([y0, y1]), np.concatenate([P, P])
 
 
-def _rhs_magnitude(o, P, y):
-    """Per RHS component, the sum of the magnitudes of the terms it adds up. A species rate is
-    sum_r nu_kr (fwd_r - rev_r) pm_r, whose terms cancel near equilibrium, so rounding (any
-    summation order) moves it by a few ulps of this magnitude, not of the result."""
-    M = o.mech
-    K, R = len(M.species), len(M.reactions)
-    N = np.zeros((K, R))
-    for r, rx in enumerate(M.reactions):
-        for k, nu in list(rx.reactants) + list(rx.products):
-            N[k, r] += abs(nu)
-    rates = o.rates(P, y)
-    mag_w = N @ ((np.abs(rates["fwd"]) + np.abs(rates["rev"])) * np.abs(rates["presmod"]))
-    W = np.asarray(M.W)
-    rho = float(rates["conc"] @ W)
-    T = y[0]
-    a = np.where((T <= np.asarray(M.tmid))[:, None], M.nasa_lo, M.nasa_hi)
-    hRT = a[:, 0] + T * (a[:, 1] / 2 + T * (a[:, 2] / 3 + T * (a[:, 3] / 4 + T * a[:, 4] / 5))) + a[:, 5] / T
-    cpR = a[:, 0] + T * (a[:, 1] + T * (a[:, 2] + T * (a[:, 3] + T * a[:, 4])))
-    Yf = np.append(y[1:], 1.0 - y[1:].sum())
-    cp = (Yf * cpR / W).sum() * 8314.46
-    mag = np.empty(K)
-    mag[0] = (np.abs(hRT) * 8314.46 * T * mag_w).sum() / (rho * cp)
-    mag[1:] = mag_w[:-1] * W[:-1] / rho
-    return mag
-
-
-def _last_species_spread(o, P, y, ulps=8):
-    """Per RHS component, how far the oracle's RHS moves when the eliminated species' mass
-    fraction, 1 - sum(Y), moves by `ulps` units of 1.0: the state carries T and the first K-1
-    mass fractions, the kernels form that difference in another summation order than the
-    oracle, and where the last species is a minor one (CH3CHO in GRI-3.0, A4 in nDodecane) its
-    reactions make the RHS sensitive to those last bits. The N2 mass fraction is moved instead
-    (one ulp of N2, about 0.7, is one ulp of 1.0 in the difference)."""
-    n2 = 1 + [sp.upper() for sp in o.mech.species].index("N2")
-    f0 = o.rhs(P, y)
-    spread = np.zeros_like(f0)
-    for d in (np.inf, -np.inf):
-        yy = y.copy()
-        for _ in range(ulps):
-            yy[n2] = np.nextafter(yy[n2], d)
-        spread = np.maximum(spread, np.abs(o.rhs(P, yy) - f0))
-    return spread
-
-
 def _component_check(m, o, y, P):
     """RHS: within 1e-9 of the oracle's value, 1e-13 of the component's term magnitude, or
-    twice the spread an 8-ulp change of the eliminated species' mass fraction causes.
+    twice the spread an 8-ulp change of the eliminated species' mass fraction causes
+    (tests/rhs_bars.py).
     Jacobian: within 1e-9 of the largest entry of its column (the wave kernel's bar)."""
+    from rhs_bars import rhs_error
     f = m.rhs(y, P, cell_layout=2)
     assert np.all(np.isfinite(f))
-    ref = np.stack([o.rhs(P[i], y[i]) for i in range(len(y))])
-    mag = np.stack([_rhs_magnitude(o, P[i], y[i]) for i in range(len(y))])
-    last = np.stack([_last_species_spread(o, P[i], y[i]) for i in range(len(y))])
-    err = np.abs(f - ref) / (1e-9 * np.abs(ref) + 1e-13 * mag + 2.0 * last + 1e-300)
+    err, ref = rhs_error(o, P, y, f)
     i, k = np.unravel_index(err.argmax(), err.shape)
     print(f"RHS: worst {err.max():.3e} of the bar at state {i} component {k}: gpu {f[i, k]:.17g} oracle "
-          f"{ref[i, k]:.17g} magnitude {mag[i, k]:.3e} y {y[i, k]:.6e} T {y[i, 0]:.2f}; states over the bar "
-          f"{sorted(set(np.nonzero(err >= 1.0)[0].tolist()))}", flush=True)
+          f"{ref[i, k]:.17g} y {y[i, k]:.6e} T {y[i, 0]:.2f}", flush=True)
     rhs_ok = err.max() < 1.0
     if m.n_state < 64:
         fw = m.rhs(y, P, cell_layout=1)
-        ew = np.abs(fw - ref) / (1e-9 * np.abs(ref) + 1e-13 * mag + 2.0 * last + 1e-300)
-        print(f"RHS of the wave kernel on the same states: worst {ew.max():.3e} of the bar at "
-              f"{np.unravel_index(ew.argmax(), ew.shape)}; block - wave at the block's worst: {f[i, k] - fw[i, k]:.3e}",
-              flush=True)
+        ew, _ = rhs_error(o, P, y, fw)
+        print(f"RHS of the wave kernel on the same states: worst {ew.max():.3e} of the bar", flush=True)
     J = m.jac(y, P, cell_layout=2)
     for i in range(len(y)):
         Jr = o.jac(P[i], y[i])

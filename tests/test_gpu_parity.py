@@ -306,3 +306,26 @@ def test_failure_codes_match_oracle(gri, lu_precision):
     yg, sg, _ = m.integrate(y0, P, np.full(8, 1e-6), atol=0.0, lu_precision=lu_precision)
     yo, so, _ = o.integrate(P, np.full(8, 1e-6), y0, atol=0.0, nthreads=8)
     assert so.tolist() == [-22] * 8 and sg.tolist() == so.tolist()
+
+
+@pytest.mark.parametrize("lu_precision,cell_layout", [(0, 1), (1, 1), (0, 2)])
+def test_inert_cells(gri, lu_precision, cell_layout):
+    """Cells with no reaction to run (pure N2, pure AR, N2/AR; 300-2500 K): the RHS is exactly
+    zero, so every error estimate is zero and the step-size rule meets pow(0, 1/L) = 0 (eta =
+    1/ADDON, then the growth limit). The GPU keeps the state bit for bit and takes the
+    oracle's steps."""
+    m, o = gri
+    sp, K = m.species, m.n_species
+    ys = []
+    for T, comp in ((300.0, {"N2": 1.0}), (1500.0, {"N2": 1.0}), (2500.0, {"AR": 1.0}), (1200.0, {"N2": 0.7, "AR": 0.3})):
+        Y = np.zeros(K)
+        for k, v in comp.items():
+            Y[sp.index(k)] = v
+        ys.append(np.concatenate([[T], Y[:-1]]))
+    y = np.array(ys)
+    P, dt = np.full(len(y), 101325.0), np.full(len(y), 1e-4)
+    yo, so, sto = o.integrate(P, dt, y, nthreads=1)
+    yg, sg, stg = m.integrate(y, P, dt, lu_precision=lu_precision, cell_layout=cell_layout)
+    assert (so == 0).all() and (sg == 0).all()
+    assert np.array_equal(yg, y) and np.array_equal(yo, y)
+    assert np.array_equal(stg["nst"], sto["nst"]), (stg["nst"], sto["nst"])
