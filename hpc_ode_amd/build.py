@@ -38,6 +38,7 @@ HOST_FLAGS = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", 
 SOURCES = [
     ("yaml_lite.cpp", HOST_FLAGS),
     ("xml_lite.cpp", HOST_FLAGS),
+    ("cti_lite.cpp", HOST_FLAGS),
     ("mechanism.cpp", HOST_FLAGS),
     ("capi.cpp", HOST_FLAGS),
     ("server.cpp", HOST_FLAGS),

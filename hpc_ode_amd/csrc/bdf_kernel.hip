@@ -156,12 +156,18 @@ struct WaveLayout {
 
 // Small mechanisms (at most 16 state variables: a 16-register Newton row, one or two reaction
 // passes) fit several waves per SIMD, which hides the latency one wave exposes. On 262,144
-// H2/air cells: 918 ms FP32 / 915 ms FP64 at two waves, 861 / 903 ms at three (168 registers,
-// a few hundred bytes of spill per lane) and 990 / 1,170 ms at four. Two is the default: one
-// three-wave FP64 build returned convergence failures for every cell after an unrelated,
-// never-taken branch was added to the cell loop, and no cause has been shown for it.
+// H2/air cells (second substep, 17 steps per cell): 28.8 ms FP32 / 29.8 ms FP64 at two waves,
+// 27.1 / 29.7 ms at three (168 registers), with identical results.
+//
+// An earlier three-wave FP64 build returned convergence failures for every cell once a
+// never-taken branch was added to the cell loop. That build wrote the LU's permutation with an
+// inline-asm v_writelane, which writes its lane whatever EXEC holds; as an asm operand the
+// register allocator could give its register to a value still live in lanes outside EXEC,
+// and the tighter three-wave register budget makes such sharing likelier. The write is now a
+// compare and select (wli), and the same three-wave build with a never-taken branch of the
+// same form (a per-workgroup exit test before each queue pull) matches the oracle.
 #ifndef HPO_WAVES_PER_EU_SMALL
-#define HPO_WAVES_PER_EU_SMALL 2
+#define HPO_WAVES_PER_EU_SMALL 3
 #endif
 template <int NS, bool LU32>
 constexpr int bdf_waves_per_eu() {

@@ -10,6 +10,7 @@
 #include <sstream>
 #include <stdexcept>
 
+#include "cti_lite.hpp"
 #include "xml_lite.hpp"
 #include "yaml_lite.hpp"
 
@@ -75,6 +76,27 @@ Side parse_side(std::string s) {
     pos = nx + 3;
   }
   return out;
+}
+
+// "A + B <=> C" / "A => B" / "A = B" -> the two sides; false for an irreversible arrow.
+bool split_equation(const std::string& eq, std::string& lhs, std::string& rhs) {
+  size_t p;
+  if ((p = eq.find("<=>")) != std::string::npos) {
+    lhs = eq.substr(0, p);
+    rhs = eq.substr(p + 3);
+    return true;
+  }
+  if ((p = eq.find("=>")) != std::string::npos) {
+    lhs = eq.substr(0, p);
+    rhs = eq.substr(p + 2);
+    return false;
+  }
+  if ((p = eq.find('=')) != std::string::npos) {
+    lhs = eq.substr(0, p);
+    rhs = eq.substr(p + 1);
+    return true;
+  }
+  throw std::runtime_error("bad equation '" + eq + "'");
 }
 
 std::vector<std::pair<int, double>> merge(const std::vector<std::pair<std::string, double>>& t,
@@ -175,20 +197,7 @@ Mechanism load_mechanism_yaml(const std::string& path) {
     Reaction rx;
     rx.equation = rd["equation"].as_string();
     std::string eq = rx.equation, lhs, rhs;
-    size_t p;
-    if ((p = eq.find("<=>")) != std::string::npos) {
-      lhs = eq.substr(0, p);
-      rhs = eq.substr(p + 3);
-    } else if ((p = eq.find("=>")) != std::string::npos) {
-      lhs = eq.substr(0, p);
-      rhs = eq.substr(p + 2);
-      rx.reversible = false;
-    } else if ((p = eq.find('=')) != std::string::npos) {
-      lhs = eq.substr(0, p);
-      rhs = eq.substr(p + 1);
-    } else {
-      throw std::runtime_error("bad equation '" + eq + "'");
-    }
+    rx.reversible = split_equation(eq, lhs, rhs);
     if (rd.has("orders")) throw std::runtime_error("non-standard reaction orders unsupported: " + eq);
     Side L = parse_side(lhs), Rr = parse_side(rhs);
     rx.reactants = merge(L.terms, index);
@@ -437,12 +446,212 @@ Mechanism load_mechanism_xml(const std::string& path) {
   return m;
 }
 
+// ---------------------------------------------------------------------------
+// Cantera CTI files (resources/mechanisms/*.cti of the reference): units(...), the first
+// ideal_gas(...) phase, species(...) with NASA7 thermo, reaction(...), three_body_reaction(...)
+// and falloff_reaction(...) with Lindemann or Troe falloff. Units follow the file's units()
+// statement; without one, Cantera's SI defaults (m, kmol, s, J/kmol).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct CtiUnits {
+  double conc = 1.0;   // (length^3 / quantity) -> m^3/kmol
+  double ea = 1.0;     // activation-energy unit -> J/kmol
+};
+
+CtiUnits cti_units(const CNode* u) {
+  CtiUnits out;
+  if (!u) return out;
+  double lf = 1.0, qf = 1.0;
+  if (const CNode* l = u->kw("length")) {
+    const std::string& s = l->as_string();
+    lf = s == "cm" ? 1e-2 : s == "mm" ? 1e-3 : s == "m" ? 1.0 : throw std::runtime_error("length unit " + s);
+  }
+  if (const CNode* q = u->kw("quantity")) {
+    const std::string& s = q->as_string();
+    qf = s == "mol" ? 1e-3 : s == "kmol" ? 1.0 : s == "molec" ? 1.0 / 6.02214076e26
+                                                  : throw std::runtime_error("quantity unit " + s);
+  }
+  out.conc = lf * lf * lf / qf;
+  if (const CNode* e = u->kw("act_energy")) out.ea = ea_factor(e->as_string());
+  if (const CNode* t = u->kw("time"))
+    if (t->as_string() != "s") throw std::runtime_error("time unit " + t->as_string());
+  return out;
+}
+
+// [A, b, E] (E a number or (value, unit)) or Arrhenius(A=, b=, E=), converted to SI-kmol.
+void cti_arrhenius(const CNode* r, double order, const CtiUnits& u, double& lnA, double& b, double& th, double& A) {
+  if (!r) throw std::runtime_error("missing rate expression");
+  const CNode *a = nullptr, *bb = nullptr, *e = nullptr;
+  if (r->kind == CNode::List || r->kind == CNode::Tuple) {
+    a = r->arg(0);
+    bb = r->arg(1);
+    e = r->arg(2);
+  } else if (r->kind == CNode::Call && r->str == "Arrhenius") {
+    a = r->kw("A") ? r->kw("A") : r->arg(0);
+    bb = r->kw("b") ? r->kw("b") : r->arg(1);
+    e = r->kw("E") ? r->kw("E") : r->arg(2);
+  } else {
+    throw std::runtime_error("unsupported rate expression");
+  }
+  if (!a) throw std::runtime_error("Arrhenius without A");
+  A = a->as_double() * std::pow(u.conc, order - 1.0);
+  b = bb ? bb->as_double() : 0.0;
+  double ef = u.ea, ev = 0.0;
+  if (e && (e->kind == CNode::Tuple || e->kind == CNode::List)) {
+    ev = e->arg(0)->as_double();
+    if (e->arg(1)) ef = ea_factor(e->arg(1)->as_string());
+  } else if (e) {
+    ev = e->as_double();
+  }
+  th = ev * ef / kRGas;
+  lnA = A > 0 ? std::log(A) : -INFINITY;
+}
+
+}  // namespace
+
+Mechanism load_mechanism_cti(const std::string& path) {
+  std::vector<std::unique_ptr<CNode>> st = cti_load_file(path);
+  Mechanism m;
+  m.path = path;
+  const CNode* units = nullptr;
+  const CNode* phase = nullptr;
+  std::map<std::string, const CNode*> spdata;
+  for (auto& s : st) {
+    if (s->str == "units" && !units) units = s.get();
+    else if (s->str == "ideal_gas" && !phase) phase = s.get();
+    else if (s->str == "species")
+      if (const CNode* n = s->kw("name")) spdata[n->as_string()] = s.get();
+  }
+  if (!phase) throw std::runtime_error("no ideal_gas phase in " + path);
+  const CtiUnits u = cti_units(units);
+  const CNode* spl = phase->kw("species");
+  if (!spl) throw std::runtime_error("phase without species");
+  std::map<std::string, int> index;
+  for (const std::string& n : words(spl->as_string())) {
+    if (n.find(':') != std::string::npos) throw std::runtime_error("species from other files unsupported: " + n);
+    index[n] = static_cast<int>(m.species.size());
+    m.species.push_back(n);
+  }
+  const int K = m.K();
+  m.W.assign(K, 0.0);
+  m.tmid.assign(K, 0.0);
+  m.nasa.assign(static_cast<size_t>(K) * 14, 0.0);
+  for (int k = 0; k < K; ++k) {
+    auto it = spdata.find(m.species[k]);
+    if (it == spdata.end()) throw std::runtime_error("species '" + m.species[k] + "' has no entry");
+    const CNode& s = *it->second;
+    const CNode* at = s.kw("atoms");
+    double w = 0;
+    if (at)
+      for (auto& kv : name_values(at->as_string())) w += atomic_weight(kv.first) * kv.second;
+    m.W[k] = w;
+    const CNode* th = s.kw("thermo");
+    if (!th) throw std::runtime_error(m.species[k] + ": no thermo");
+    std::vector<const CNode*> blocks;
+    if (th->kind == CNode::Call) blocks.push_back(th);
+    else
+      for (auto& b : th->items) blocks.push_back(b.get());
+    for (const CNode* b : blocks)
+      if (b->kind != CNode::Call || b->str != "NASA" || !b->arg(0) || !b->arg(1) || b->arg(0)->items.size() != 2 ||
+          b->arg(1)->items.size() != 7)
+        throw std::runtime_error(m.species[k] + ": only NASA7 thermo is supported");
+    std::sort(blocks.begin(), blocks.end(), [](const CNode* a, const CNode* b) {
+      return a->arg(0)->arg(0)->as_double() < b->arg(0)->arg(0)->as_double();
+    });
+    const CNode* lo = blocks.front();
+    const CNode* hi = blocks.back();
+    m.tmid[k] = lo->arg(0)->arg(1)->as_double();
+    for (int j = 0; j < 7; ++j) {
+      m.nasa[14 * k + j] = lo->arg(1)->arg(j)->as_double();
+      m.nasa[14 * k + 7 + j] = hi->arg(1)->arg(j)->as_double();
+    }
+  }
+  const CNode* rsel = phase->kw("reactions");
+  if (!rsel) return m;
+  if (rsel->kind != CNode::Str || rsel->str != "all")
+    throw std::runtime_error("only reactions = 'all' is supported");
+  for (auto& sp : st) {
+    const CNode& s = *sp;
+    const bool elem = s.str == "reaction", three = s.str == "three_body_reaction", fall = s.str == "falloff_reaction";
+    if (!elem && !three && !fall) {
+      if (s.str == "pdep_arrhenius" || s.str == "chebyshev_reaction" || s.str == "chemically_activated_reaction" ||
+          s.str == "surface_reaction" || s.str == "edge_reaction")
+        throw std::runtime_error(s.str + " unsupported");
+      continue;
+    }
+    Reaction rx;
+    const CNode* eqn = s.kw("equation") ? s.kw("equation") : s.arg(0);
+    if (!eqn) throw std::runtime_error("reaction without equation");
+    rx.equation = eqn->as_string();
+    if (s.kw("order")) throw std::runtime_error("non-standard reaction orders unsupported: " + rx.equation);
+    std::string lhs, rhs;
+    rx.reversible = split_equation(rx.equation, lhs, rhs);
+    Side L = parse_side(lhs), Rr = parse_side(rhs);
+    rx.reactants = merge(L.terms, index);
+    rx.products = merge(Rr.terms, index);
+    double order = 0;
+    for (auto& kv : rx.reactants) order += kv.second;
+    std::vector<std::pair<int, double>> effs;
+    if (const CNode* e = s.kw("efficiencies"))
+      for (auto& kv : name_values(e->as_string())) {
+        auto it = index.find(kv.first);
+        if (it != index.end()) effs.emplace_back(it->second, kv.second);
+      }
+    std::sort(effs.begin(), effs.end());
+    const double deff = s.kw("default_efficiency") ? s.kw("default_efficiency")->as_double() : 1.0;
+    auto set_eff = [&](double base, const std::vector<std::pair<int, double>>& e) {
+      rx.eff_base = base;
+      rx.eff.clear();
+      for (auto& kv : e) rx.eff.emplace_back(kv.first, kv.second - base);
+    };
+    const CNode* kf = s.kw("kf") ? s.kw("kf") : s.arg(1);
+    if (elem) {
+      rx.kind = kElementary;
+      cti_arrhenius(kf, order, u, rx.lnA, rx.b, rx.theta, rx.A);
+    } else if (three) {
+      rx.kind = kThreeBody;
+      cti_arrhenius(kf, order + 1, u, rx.lnA, rx.b, rx.theta, rx.A);
+      set_eff(deff, effs);
+    } else {
+      cti_arrhenius(kf, order, u, rx.lnA, rx.b, rx.theta, rx.A);
+      cti_arrhenius(s.kw("kf0") ? s.kw("kf0") : s.arg(2), order + 1, u, rx.lnA0, rx.b0, rx.theta0, rx.A0);
+      if (!L.collider.empty() && L.collider != "M") {
+        auto it = index.find(L.collider);
+        if (it == index.end()) throw std::runtime_error("unknown collider " + L.collider);
+        set_eff(0.0, {{it->second, 1.0}});
+      } else {
+        set_eff(deff, effs);
+      }
+      const CNode* f = s.kw("falloff");
+      if (!f || (f->kind == CNode::Call && f->str == "Lindemann")) {
+        rx.kind = kLindemann;
+      } else if (f->kind == CNode::Call && f->str == "Troe") {
+        rx.kind = kTroe;
+        const char* names[4] = {"A", "T3", "T1", "T2"};
+        for (int j = 0; j < 4; ++j) {
+          const CNode* v = f->kw(names[j]) ? f->kw(names[j]) : f->arg(static_cast<size_t>(j));
+          if (!v && j < 3) throw std::runtime_error("Troe falloff needs A, T3 and T1: " + rx.equation);
+          if (j == 3) rx.troe_has_t2 = v != nullptr;
+          rx.troe[j] = v ? v->as_double() : 0.0;
+        }
+      } else {
+        throw std::runtime_error("falloff type '" + (f->kind == CNode::Call ? f->str : std::string("?")) +
+                                 "' unsupported: " + rx.equation);
+      }
+    }
+    m.reactions.push_back(std::move(rx));
+  }
+  return m;
+}
+
 Mechanism load_mechanism(const std::string& path) {
   auto ends = [&](const char* e) {
     const size_t n = std::char_traits<char>::length(e);
     return path.size() >= n && path.compare(path.size() - n, n, e) == 0;
   };
   if (ends(".xml") || ends(".ctml")) return load_mechanism_xml(path);
+  if (ends(".cti")) return load_mechanism_cti(path);
   return load_mechanism_yaml(path);
 }
 

@@ -48,7 +48,8 @@ struct Mechanism {
 
 Mechanism load_mechanism_yaml(const std::string& path);
 Mechanism load_mechanism_xml(const std::string& path);   // Cantera CTML
-Mechanism load_mechanism(const std::string& path);       // by extension: .xml/.ctml or YAML
+Mechanism load_mechanism_cti(const std::string& path);   // Cantera CTI
+Mechanism load_mechanism(const std::string& path);       // by extension: .xml/.ctml, .cti or YAML
 
 constexpr double kGasConstant = 8314.4621;   // J/(kmol K), the value the kernels use
 

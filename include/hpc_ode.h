@@ -86,9 +86,9 @@ void hpo_opts_default(hpo_opts *o);
 
 /* ---- mechanism: replaces the compiled-in pyJac code (src/Mechanism/CPU, src/Mechanism/GPU) ---- */
 int hpo_mech_load_yaml(const char *path, hpo_mech **out);
-/* Cantera YAML, or Cantera CTML/XML for a path ending in .xml or .ctml (the reference ships
- * both forms in resources/mechanisms; e.g. h2_li_2004.xml, the mechanism of
- * ref_data/res_h2.csv, ref_data/README.md:15). */
+/* Cantera YAML, Cantera CTML/XML for a path ending in .xml or .ctml, or Cantera CTI for a path
+ * ending in .cti (the reference ships all three forms in resources/mechanisms; e.g.
+ * h2_li_2004.xml, the mechanism of ref_data/res_h2.csv, ref_data/README.md:15). */
 int hpo_mech_load(const char *path, hpo_mech **out);
 void hpo_mech_destroy(hpo_mech *m);
 /* n_state = n_species (T plus n_species - 1 mass fractions) */
