@@ -22,8 +22,11 @@
 #ifndef HPO_BLK_WAVES_PER_EU_FP64
 #define HPO_BLK_WAVES_PER_EU_FP64 1
 #endif
+// Two waves per SIMD for the FP32 Newton matrix (a 100-float row fits the 256 registers):
+// two workgroups per CU instead of one, 65,536 GRI cells in 2,452 ms against 3,995 ms and
+// 16,384 nDodecane cells in 2,434 ms against 2,718 ms (first substep).
 #ifndef HPO_BLK_WAVES_PER_EU_FP32
-#define HPO_BLK_WAVES_PER_EU_FP32 1
+#define HPO_BLK_WAVES_PER_EU_FP32 2
 #endif
 
 namespace hpo {
@@ -35,39 +38,87 @@ constexpr int kBlkThreads = 64 * kBlkWaves;
 struct BlkTabs {
   const double* sp;        // [K][kSpStride]
   const BlkRxn* rxn;       // [R]
-  const double* arr;       // [R][3]
+  double* arr;             // [R][3] (in LDS: the PLOG rows are rewritten for each cell)
   const PdepRec* pd;
   const double* effv;
   const uint8_t* effs;
   const uint16_t* csr_ptr; // [K+1]
-  const uint32_t* csr;
+  const double* coef;      // distinct net stoichiometric coefficients
+  const uint32_t* csr;     // r | coefficient index << 16
   const double* exptab;
   const uint8_t* slot_sp;  // global
   const uint32_t* col_ptr; // global
   const uint32_t* col;     // global
-  int K, R;
+  const uint16_t* plog_rxn;   // global: device reaction of each PLOG rate
+  const uint16_t* plog_ptr;   // global: its levels
+  const double* plog_lv;      // global: [level][4] ln P, lnA, b, theta
+  const uint16_t* lump_ptr;   // global: products of lumped reactions
+  const uint32_t* lump;       // global: species | coefficient index << 16
+  int K, R, n_plog;
   int conv;                // reactor: 0 constant pressure (P is the pressure), 1 constant volume
                            // (P is the density)
 };
 
-HPO_DI BlkTabs make_blk_tabs(const uint8_t* lds, const uint8_t* glob, const BlkLayout& L) {
+// Tables at offsets below lds_bytes were staged into LDS; the rest are read from global memory.
+HPO_DI BlkTabs make_blk_tabs(uint8_t* lds, const uint8_t* glob, const BlkLayout& L) {
+  auto at = [&](int32_t off) -> const uint8_t* { return off < L.lds_bytes ? lds + off : glob + off; };
   BlkTabs t;
-  t.sp = reinterpret_cast<const double*>(lds + L.off_species);
-  t.rxn = reinterpret_cast<const BlkRxn*>(lds + L.off_rxn);
-  t.arr = reinterpret_cast<const double*>(lds + L.off_arr);
-  t.pd = reinterpret_cast<const PdepRec*>(lds + L.off_pdep);
-  t.effv = reinterpret_cast<const double*>(lds + L.off_eff_val);
-  t.effs = lds + L.off_eff_sp;
-  t.csr_ptr = reinterpret_cast<const uint16_t*>(lds + L.off_csr_ptr);
-  t.csr = reinterpret_cast<const uint32_t*>(lds + L.off_csr);
+  t.sp = reinterpret_cast<const double*>(at(L.off_species));
+  t.rxn = reinterpret_cast<const BlkRxn*>(at(L.off_rxn));
+  t.arr = reinterpret_cast<double*>(lds + L.off_arr);   // always staged
+  t.pd = reinterpret_cast<const PdepRec*>(at(L.off_pdep));
+  t.effv = reinterpret_cast<const double*>(at(L.off_eff_val));
+  t.effs = at(L.off_eff_sp);
+  t.csr_ptr = reinterpret_cast<const uint16_t*>(at(L.off_csr_ptr));
+  t.coef = reinterpret_cast<const double*>(at(L.off_coef));
+  t.csr = reinterpret_cast<const uint32_t*>(at(L.off_csr));
   t.exptab = reinterpret_cast<const double*>(lds + L.off_exptab);
   t.slot_sp = glob + L.off_slot_sp;
   t.col_ptr = reinterpret_cast<const uint32_t*>(glob + L.off_col_ptr);
   t.col = reinterpret_cast<const uint32_t*>(glob + L.off_col);
+  t.plog_rxn = reinterpret_cast<const uint16_t*>(glob + L.off_plog_rxn);
+  t.plog_ptr = reinterpret_cast<const uint16_t*>(glob + L.off_plog_ptr);
+  t.plog_lv = reinterpret_cast<const double*>(glob + L.off_plog_lv);
+  t.lump_ptr = reinterpret_cast<const uint16_t*>(glob + L.off_lump_ptr);
+  t.lump = reinterpret_cast<const uint32_t*>(glob + L.off_lump);
   t.K = L.K;
   t.R = L.R;
+  t.n_plog = L.n_plog;
   t.conv = 0;
   return t;
+}
+
+// PLOG rates at the cell's pressure P (Pa): each is an Arrhenius expression whose lnA, b and
+// theta are those of the pressure level at or below P and the one above it, interpolated
+// linearly in ln P (the interpolation of ln k is then exact for every T); below the first level
+// or above the last, that level's. Written into the staged Arrhenius table; the caller orders
+// the writes before the rates are read.
+HPO_DI void blk_plog_cell(const BlkTabs& t, double P, int tid) {
+  if (t.n_plog == 0) return;
+  const double lnP = log(P);
+  for (int i = tid; i < t.n_plog; i += kBlkThreads) {
+    const int b = t.plog_ptr[i], e = t.plog_ptr[i + 1];
+    const double* lv = t.plog_lv;
+    int j = b;
+    double w = 0.0;
+    if (lnP > lv[4 * b] && lnP < lv[4 * (e - 1)]) {
+      while (j + 2 < e && lv[4 * (j + 1)] <= lnP) ++j;
+      w = (lnP - lv[4 * j]) / (lv[4 * (j + 1)] - lv[4 * j]);
+    } else if (lnP >= lv[4 * (e - 1)]) {
+      j = e - 1;
+    }
+    const double* a = lv + 4 * j;
+    double lnA = a[1], bb = a[2], th = a[3];
+    if (w != 0.0) {
+      lnA += w * (a[5] - a[1]);
+      bb += w * (a[6] - a[2]);
+      th += w * (a[7] - a[3]);
+    }
+    double* dst = t.arr + 3 * t.plog_rxn[i];
+    dst[0] = lnA;
+    dst[1] = bb;
+    dst[2] = th;
+  }
 }
 
 // Workgroup LDS work area (after the tables). Sizes: NB = kBlkThreads.
@@ -111,11 +162,18 @@ HPO_DI BlkLds make_blk_lds(double* base, int K, int R) {
 
 // Per-workgroup scratch (doubles): d(phi)/dC slots, per-reaction values, d(wdot)/dC rows,
 // the Jacobian rows of the state.
+// States of more than kBlkRegMax variables keep the Newton matrix in the workgroup's scratch
+// (column j of the thread rows at M[j * kBlkThreads + t], so a column is one coalesced access)
+// instead of one register row per thread.
+constexpr int kBlkRegMax = 128;
+
 struct BlkScratch {
-  double *dphi, *gphi, *gdT, *gdm, *dH, *JC, *J;
+  double *dphi, *gphi, *gdT, *gdm, *dH, *JC, *J, *M;
 };
 __host__ __device__ inline int64_t blk_scratch_doubles(const BlkLayout& L, int NS) {
-  int64_t o = ((L.n_slots + 1) & ~1) + 4 * ((L.R + 1) & ~1) + static_cast<int64_t>(L.K) * L.K + static_cast<int64_t>(NS) * NS;
+  int64_t o = ((L.n_slots + 1) & ~1) + 4 * ((L.R + 1) & ~1) + static_cast<int64_t>(L.K) * L.K +
+              ((static_cast<int64_t>(NS) * NS + 1) & ~int64_t(1));
+  if (NS > kBlkRegMax) o += static_cast<int64_t>(NS) * kBlkThreads;
   return (o + 31) & ~int64_t(31);
 }
 HPO_DI BlkScratch make_blk_scratch(double* base, const BlkLayout& L) {
@@ -127,7 +185,8 @@ HPO_DI BlkScratch make_blk_scratch(double* base, const BlkLayout& L) {
   s.gdm = p; p += (L.R + 1) & ~1;
   s.dH = p; p += (L.R + 1) & ~1;
   s.JC = p; p += static_cast<int64_t>(L.K) * L.K;
-  s.J = p;
+  s.J = p; p += (static_cast<int64_t>(L.K) * L.K + 1) & ~int64_t(1);
+  s.M = p;   // only when K > kBlkRegMax
   return s;
 }
 
@@ -319,7 +378,7 @@ HPO_DI double blk_rhs(const BlkTabs& t, const BlkLds& w, double y, double P, int
     const int b = t.csr_ptr[tid], e = t.csr_ptr[tid + 1];
     for (int i = b; i < e; ++i) {
       const uint32_t c = t.csr[i];
-      wdot += static_cast<double>(static_cast<int8_t>(c >> 16)) * w.phi[c & 0xFFFFu];
+      wdot += t.coef[c >> 16] * w.phi[c & 0xFFFFu];
     }
   }
   const bool sp = tid < K;
@@ -415,6 +474,15 @@ HPO_DI void blk_jacobian(const BlkTabs& t, const BlkLds& w, const BlkScratch& g,
     g.gphi[r] = dq * pm;
     g.gdm[r] = fM * eff_base;
     g.gdT[r] = pm * (fwd * dlnkf - rev * dlnkr) + dq * dpm_dT;
+    if (h.flags & kBlkLumped) {
+      // an irreversible reaction whose products are not in the slots: their enthalpy and moles
+      for (int i = t.lump_ptr[r]; i < t.lump_ptr[r + 1]; ++i) {
+        const uint32_t c = t.lump[i];
+        const double nu = t.coef[c >> 16];
+        dHRT += nu * w.hRT[c & 0xFFFFu];
+        dnu += nu;
+      }
+    }
     g.dH[r] = (dHRT - t.conv * dnu) * RT;   // reaction enthalpy, or internal energy at constant volume
   }
   __syncthreads();
@@ -433,7 +501,7 @@ HPO_DI void blk_jacobian(const BlkTabs& t, const BlkLds& w, const BlkScratch& g,
     for (int i = b; i < e; ++i) {
       const uint32_t c = t.csr[i];
       const int r = static_cast<int>(c & 0xFFFFu);
-      const double nu = static_cast<double>(static_cast<int8_t>(c >> 16));
+      const double nu = t.coef[c >> 16];
       wdot += nu * g.gphi[r];
       JT += nu * g.gdT[r];
       Jm += nu * g.gdm[r];
@@ -586,16 +654,83 @@ HPO_DI double blk_gj_apply(const BlkInv<NS, F>& L, double b, const BlkLds& w, in
   return tid < NS ? w.vec[tid] : 0.0;
 }
 
+// The same elimination with the matrix in the workgroup's scratch (M[j * kBlkThreads + t] holds
+// row t, column j; states of more than kBlkRegMax variables): thread t updates its row column by
+// column, and the pivot row reaches LDS one column per thread. step: as BlkInv::step.
+template <int NS, typename F>
+HPO_DI bool blk_gj_invert_g(F* M, int& step, const BlkLds& w, int tid) {
+  bool done = tid >= NS;
+  bool singular = false;
+  F rd = F(0);
+  step = -1;
+  F* prow = reinterpret_cast<F*>(w.prow);
+  for (int k = 0; k < NS; ++k) {
+    const F akk = M[k * kBlkThreads + tid];
+    const unsigned key = done ? 0u : blk_pivot_key(akk, tid);
+    const unsigned kmax = bmax_u(key, w.iscal, tid);   // its barriers also end the last step's reads
+    const int p = 255 - static_cast<int>(kmax & 255u);
+    if (tid < NS) prow[tid] = M[tid * kBlkThreads + p];
+    if (tid == p) w.perm[k] = p;
+    __syncthreads();
+    const F piv = prow[k];
+    const bool bad = (kmax == 0u) || !(fabs(static_cast<double>(piv)) > 0.0);
+    singular = singular || bad;
+    const F mult = bad ? F(0) : F(1) / piv;
+    const bool isp = tid == p;
+    if (isp && !bad) {
+      done = true;
+      rd = mult;
+      step = k;
+    }
+    const F l = isp ? F(0) : akk * mult;
+    if (tid < NS) {
+      for (int j = 0; j < NS; ++j) {
+        if (j == k) continue;
+        M[j * kBlkThreads + tid] = fma(-l, prow[j], M[j * kBlkThreads + tid]);
+      }
+      M[k * kBlkThreads + tid] = isp ? F(1) : -l;
+    }
+  }
+  if (tid < NS)
+    for (int j = 0; j < NS; ++j) M[j * kBlkThreads + tid] *= rd;
+  __syncthreads();
+  return !singular;
+}
+
+template <int NS, typename F>
+HPO_DI double blk_gj_apply_g(const F* M, int step, double b, const BlkLds& w, int tid) {
+  __syncthreads();   // previous readers of vec / vec2 are done
+  if (step >= 0) w.vec2[step] = b;
+  __syncthreads();
+  F x0 = F(0), x1 = F(0);
+  if (tid < NS) {
+    int j = 0;
+    for (; j + 1 < NS; j += 2) {
+      x0 = fma(M[j * kBlkThreads + tid], static_cast<F>(w.vec2[j]), x0);
+      x1 = fma(M[(j + 1) * kBlkThreads + tid], static_cast<F>(w.vec2[j + 1]), x1);
+    }
+    if (j < NS) x0 = fma(M[j * kBlkThreads + tid], static_cast<F>(w.vec2[j]), x0);
+  }
+  if (step >= 0) w.vec[step] = static_cast<double>(x0 + x1);
+  __syncthreads();
+  return tid < NS ? w.vec[tid] : 0.0;
+}
+
+struct BlkInvStep {
+  int step;
+};
+
 template <int NS, bool LU32>
 struct BlockLayout {
   using F = typename std::conditional<LU32, float, double>::type;
+  static constexpr bool kGlobalM = NS > kBlkRegMax;
   const BlkTabs& t;
   const BlkLds& w;
   const BlkScratch& g;
   const int tid;
   const int wave_lane;
   const double* exptab;
-  BlkInv<NS, F> inv;
+  typename std::conditional<kGlobalM, BlkInvStep, BlkInv<NS, F>>::type inv;
 
   HPO_DI BlockLayout(const BlkTabs& t_, const BlkLds& w_, const BlkScratch& g_, int tid_)
       : t(t_), w(w_), g(g_), tid(tid_), wave_lane(tid_ & 63), exptab(t_.exptab) {}
@@ -612,18 +747,34 @@ struct BlockLayout {
       if (PROF) cj = clock64() - c0;
     }
     const double* Jrow = g.J + static_cast<int64_t>(tid < NS ? tid : 0) * NS;
+    if constexpr (kGlobalM) {
+      F* M = reinterpret_cast<F*>(g.M);
+      for (int j = 0; j < NS; ++j) {
+        const double m = -gamma * Jrow[j] + ((j == tid) ? 1.0 : 0.0);
+        M[j * kBlkThreads + tid] = tid < NS ? static_cast<F>(m) : F(0);
+      }
+      return blk_gj_invert_g<NS, F>(M, inv.step, w, tid);
+    } else {
 #pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      const double m = -gamma * Jrow[j] + ((j == tid) ? 1.0 : 0.0);
-      inv.a[j] = tid < NS ? static_cast<F>(m) : F(0);
+      for (int j = 0; j < NS; ++j) {
+        const double m = -gamma * Jrow[j] + ((j == tid) ? 1.0 : 0.0);
+        inv.a[j] = tid < NS ? static_cast<F>(m) : F(0);
+      }
+      return blk_gj_invert<NS, F>(inv, w, tid);
     }
-    return blk_gj_invert<NS, F>(inv, w, tid);
+  }
+
+  HPO_DI double apply(double b) {
+    if constexpr (kGlobalM)
+      return blk_gj_apply_g<NS, F>(reinterpret_cast<const F*>(g.M), inv.step, b, w, tid);
+    else
+      return blk_gj_apply<NS, F>(inv, b, w, tid);
   }
 
   // x = M^-1 b; with the FP32 inverse, `sweeps` FP64 residual sweeps r = b - (I - gamma J) x
   // against the FP64 Jacobian rows in scratch.
   HPO_DI double solve(double b, double gl, int sweeps) {
-    double x = blk_gj_apply<NS, F>(inv, b, w, tid);
+    double x = apply(b);
     if constexpr (LU32) {
       const double* Jrow = g.J + static_cast<int64_t>(tid < NS ? tid : 0) * NS;
       for (int sw = 0; sw < sweeps; ++sw) {
@@ -633,7 +784,7 @@ struct BlockLayout {
         double Jx = 0.0;
         for (int j = 0; j < NS; ++j) Jx = fma(Jrow[j], w.y[j], Jx);
         const double r = (tid < NS) ? b - x + gl * Jx : 0.0;
-        x += blk_gj_apply<NS, F>(inv, r, w, tid);
+        x += apply(r);
       }
     }
     return x;
@@ -668,6 +819,8 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((
     const unsigned c = static_cast<unsigned>(w.iscal[8]);
     if (static_cast<int64_t>(c) >= A.ncells) break;
     const int64_t cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
+    blk_plog_cell(t, A.P[cell], tid);   // PLOG rates at this cell's pressure
+    __syncthreads();
     BlockLayout<NS, LU32> Ly(t, w, g, tid);
     integrate_cell<NS, LU32, PROF>(A, Ly, S, cell);
   }
@@ -687,6 +840,9 @@ __global__ __launch_bounds__(kBlkThreads) void blk_eval_kernel(EvalArgs A) {
   BlkLds w = make_blk_lds(reinterpret_cast<double*>(smem + Lay.lds_bytes), Lay.K, Lay.R);
   BlkScratch g = make_blk_scratch(A.scratch + static_cast<int64_t>(blockIdx.x) * A.scratch_stride, Lay);
   for (int64_t c = blockIdx.x; c < A.ncells; c += gridDim.x) {
+    __syncthreads();   // the previous cell's readers of the Arrhenius table are done
+    blk_plog_cell(t, A.P[c], tid);
+    __syncthreads();
     const double y = tid < NS ? A.y[c * A.ld + tid] : 0.0;
     const double P = A.P[c];
     if constexpr (MODE == 0) {
@@ -721,7 +877,11 @@ __global__ __launch_bounds__(kBlkThreads) void blk_gj_kernel(LuArgs A) {
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 // ---------------------------------------------------------------------------
-#define HPO_FOR_EACH_BLK_NS(X) X(53) X(100)
+// State sizes with a block kernel: GRI-3.0 (53), nDodecane_Reitz (100), ESTiMatE-Mech (214, the
+// Newton matrix in scratch). The Gauss-Jordan component kernel keeps its register row, so it
+// exists for the first two only.
+#define HPO_FOR_EACH_BLK_NS(X) X(53) X(100) X(214)
+#define HPO_FOR_EACH_BLK_GJ_NS(X) X(53) X(100)
 
 bool block_kernel_supports(int ns) {
 #define HPO_CASE(N) \
@@ -731,8 +891,9 @@ bool block_kernel_supports(int ns) {
   return false;
 }
 
-int blk_work_lds_bytes(const BlkLayout& L) {
-  return (dev::blk_lds_doubles(L.K, L.R) + dev::kBlkWaves * dev::kScalDoubles) * 8;
+int blk_work_lds_bytes(const BlkLayout& L) { return blk_work_lds_bytes_for(L.K, L.R); }
+int blk_work_lds_bytes_for(int K, int R) {
+  return (dev::blk_lds_doubles(K, R) + dev::kBlkWaves * dev::kScalDoubles) * 8;
 }
 int64_t blk_scratch_doubles_host(const BlkLayout& L) { return dev::blk_scratch_doubles(L, L.K); }
 
@@ -784,7 +945,7 @@ hipError_t launch_gj_block(int ns, int fp32, const LuArgs& A, int64_t ncases, hi
       hipLaunchKernelGGL((dev::blk_gj_kernel<N, double>), dim3(ncases), dim3(dev::kBlkThreads), shmem, st, A);  \
     return hipGetLastError();                                                                                    \
   }
-  HPO_FOR_EACH_BLK_NS(HPO_CASE)
+  HPO_FOR_EACH_BLK_GJ_NS(HPO_CASE)
 #undef HPO_CASE
   return hipErrorInvalidValue;
 }

@@ -148,6 +148,8 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   check_opts(o);
   const int ns = m->host.K();
   const bool block_layout = use_block_layout(m, o);
+  if (o.reactor == 1 && m->has_block && m->bblob.lay.n_plog > 0)
+    throw std::invalid_argument("PLOG rates are supported at constant pressure only");
   if (ncells < 0 || ncells > 0x7fffffffLL) throw std::invalid_argument("ncells out of range");
   if (ld < ns) throw std::invalid_argument("leading dimension smaller than the state");
   m->ensure_device();
@@ -307,7 +309,7 @@ static int mech_load(const char* path, hpo_mech** out, bool by_extension) {
       wave_err = e.what();
     }
     try {
-      m->bblob = hpo::build_block_tables(m->host);
+      m->bblob = hpo::build_block_tables(m->host, hpo::blk_work_lds_bytes_for(m->host.K(), m->host.R()));
       m->has_block = true;
     } catch (const std::exception& e) {
       block_err = e.what();
@@ -590,6 +592,8 @@ static int eval_common(const hpo_mech* mc, int32_t cell_layout, int32_t reactor,
   const bool block_ok = m->has_block && hpo::block_kernel_supports(ns);
   const bool block = cell_layout == 2 || (cell_layout == 0 && !wave_ok);
   if (block ? !block_ok : !wave_ok) return fail(HPO_E_UNSUPPORTED, "no kernel of this layout for this state size");
+  if (reactor == 1 && m->has_block && m->bblob.lay.n_plog > 0)
+    return fail(HPO_E_INVALID, "PLOG rates are supported at constant pressure only");
   if (n == 0) return HPO_OK;
   double *dy = nullptr, *dP = nullptr, *dout = nullptr;
   int rc = HPO_OK;

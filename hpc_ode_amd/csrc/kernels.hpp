@@ -85,6 +85,7 @@ hipError_t launch_eval(int ns, const EvalArgs& A, int grid, int block, size_t sh
 // Block-per-cell kernel (block_kernel.hip): one 256-thread workgroup per cell.
 bool block_kernel_supports(int ns);
 int blk_work_lds_bytes(const BlkLayout& L);           // LDS after the tables, per workgroup
+int blk_work_lds_bytes_for(int K, int R);             // the same before the tables are built
 int64_t blk_scratch_doubles_host(const BlkLayout& L); // global scratch per workgroup
 const void* bdf_block_kernel_fn(int ns, int lu32, int prof);
 hipError_t launch_bdf_block(int ns, const BdfArgs& A, int lu32, int grid, size_t shmem, hipStream_t st);

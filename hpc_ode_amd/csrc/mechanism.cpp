@@ -99,6 +99,32 @@ bool split_equation(const std::string& eq, std::string& lhs, std::string& rhs) {
   throw std::runtime_error("bad equation '" + eq + "'");
 }
 
+double pressure_factor(const std::string& u) {
+  if (u == "Pa") return 1.0;
+  if (u == "kPa") return 1e3;
+  if (u == "MPa") return 1e6;
+  if (u == "bar") return 1e5;
+  if (u == "atm") return kPAtm;
+  if (u == "torr") return kPAtm / 760.0;
+  throw std::runtime_error("pressure unit " + u);
+}
+
+// Sorts the levels by pressure; a pressure given twice (a sum of rates at one level) and a
+// non-positive pre-exponential factor are refused.
+void finish_plog(Reaction& rx) {
+  std::sort(rx.plog.begin(), rx.plog.end(), [](const PlogLevel& a, const PlogLevel& b) { return a.lnP < b.lnP; });
+  for (size_t i = 0; i < rx.plog.size(); ++i) {
+    if (!std::isfinite(rx.plog[i].lnA)) throw std::runtime_error("PLOG with a non-positive A unsupported: " + rx.equation);
+    if (i > 0 && rx.plog[i].lnP == rx.plog[i - 1].lnP)
+      throw std::runtime_error("PLOG with several rates at one pressure unsupported: " + rx.equation);
+  }
+  if (rx.plog.empty()) throw std::runtime_error("PLOG without pressure levels: " + rx.equation);
+  rx.lnA = rx.plog[0].lnA;
+  rx.A = rx.plog[0].A;
+  rx.b = rx.plog[0].b;
+  rx.theta = rx.plog[0].theta;
+}
+
 std::vector<std::pair<int, double>> merge(const std::vector<std::pair<std::string, double>>& t,
                                           const std::map<std::string, int>& index) {
   std::map<int, double> acc;
@@ -247,6 +273,22 @@ Mechanism load_mechanism_yaml(const std::string& path) {
     } else if (type == "elementary") {
       rx.kind = kElementary;
       arrhenius(rd["rate-constant"], order, cf, ef, rx.lnA, rx.b, rx.theta, rx.A);
+    } else if (type == "pressure-dependent-Arrhenius") {
+      rx.kind = kElementary;
+      const YNode& lv = rd["rate-constants"];
+      for (size_t i = 0; i < lv.size(); ++i) {
+        PlogLevel L;
+        const YNode& pn = lv[i]["P"];
+        double pv = 0.0;
+        std::istringstream ps(pn.as_string());
+        std::string unit;
+        ps >> pv;
+        if (ps >> unit) pv *= pressure_factor(unit);
+        arrhenius(lv[i], order, cf, ef, L.lnA, L.b, L.theta, L.A);
+        L.lnP = std::log(pv);
+        rx.plog.push_back(L);
+      }
+      finish_plog(rx);
     } else {
       throw std::runtime_error("reaction type '" + type + "' unsupported: " + eq);
     }
@@ -417,6 +459,17 @@ Mechanism load_mechanism_xml(const std::string& path) {
     for (auto& kv : effs) rx.eff.emplace_back(kv.first, kv.second - deff);
     if (type.empty() || type == "elementary") {
       rx.kind = kElementary;
+    } else if (type == "plog") {
+      rx.kind = kElementary;
+      for (const XNode* a : rc->all("Arrhenius")) {
+        const XNode* pe = a->child("P");
+        if (!pe) throw std::runtime_error("PLOG rate without a pressure: " + rx.equation);
+        PlogLevel L;
+        xml_arrhenius(*a, L.lnA, L.b, L.theta, L.A);
+        L.lnP = std::log(std::stod(pe->text) * pressure_factor(pe->get("units", "Pa")));
+        rx.plog.push_back(L);
+      }
+      finish_plog(rx);
     } else if (type == "threeBody") {
       rx.kind = kThreeBody;
     } else if (type == "falloff") {
@@ -479,6 +532,9 @@ CtiUnits cti_units(const CNode* u) {
   return out;
 }
 
+void cti_arrhenius3(const CNode* a, const CNode* bb, const CNode* e, double order, const CtiUnits& u, double& lnA,
+                    double& b, double& th, double& A);
+
 // [A, b, E] (E a number or (value, unit)) or Arrhenius(A=, b=, E=), converted to SI-kmol.
 void cti_arrhenius(const CNode* r, double order, const CtiUnits& u, double& lnA, double& b, double& th, double& A) {
   if (!r) throw std::runtime_error("missing rate expression");
@@ -494,6 +550,12 @@ void cti_arrhenius(const CNode* r, double order, const CtiUnits& u, double& lnA,
   } else {
     throw std::runtime_error("unsupported rate expression");
   }
+  cti_arrhenius3(a, bb, e, order, u, lnA, b, th, A);
+}
+
+// A, b and E (a number or (value, unit)) converted to SI-kmol.
+void cti_arrhenius3(const CNode* a, const CNode* bb, const CNode* e, double order, const CtiUnits& u, double& lnA,
+                    double& b, double& th, double& A) {
   if (!a) throw std::runtime_error("Arrhenius without A");
   A = a->as_double() * std::pow(u.conc, order - 1.0);
   b = bb ? bb->as_double() : 0.0;
@@ -574,9 +636,10 @@ Mechanism load_mechanism_cti(const std::string& path) {
   for (auto& sp : st) {
     const CNode& s = *sp;
     const bool elem = s.str == "reaction", three = s.str == "three_body_reaction", fall = s.str == "falloff_reaction";
-    if (!elem && !three && !fall) {
-      if (s.str == "pdep_arrhenius" || s.str == "chebyshev_reaction" || s.str == "chemically_activated_reaction" ||
-          s.str == "surface_reaction" || s.str == "edge_reaction")
+    const bool plog = s.str == "pdep_arrhenius";
+    if (!elem && !three && !fall && !plog) {
+      if (s.str == "chebyshev_reaction" || s.str == "chemically_activated_reaction" || s.str == "surface_reaction" ||
+          s.str == "edge_reaction")
         throw std::runtime_error(s.str + " unsupported");
       continue;
     }
@@ -606,7 +669,26 @@ Mechanism load_mechanism_cti(const std::string& path) {
       for (auto& kv : e) rx.eff.emplace_back(kv.first, kv.second - base);
     };
     const CNode* kf = s.kw("kf") ? s.kw("kf") : s.arg(1);
-    if (elem) {
+    if (plog) {
+      // each positional argument after the equation: [(P, unit), A, b, E]
+      rx.kind = kElementary;
+      for (size_t i = 1; i < s.items.size(); ++i) {
+        const CNode* lv = s.arg(i);
+        if (!lv || (lv->kind != CNode::List && lv->kind != CNode::Tuple) || lv->items.size() != 4)
+          throw std::runtime_error("pdep_arrhenius level must be [(P, unit), A, b, E]: " + rx.equation);
+        const CNode* pn = lv->arg(0);
+        double pv = 0.0;
+        if (pn->kind == CNode::Tuple || pn->kind == CNode::List)
+          pv = pn->arg(0)->as_double() * (pn->arg(1) ? pressure_factor(pn->arg(1)->as_string()) : 1.0);
+        else
+          pv = pn->as_double();
+        PlogLevel L;
+        cti_arrhenius3(lv->arg(1), lv->arg(2), lv->arg(3), order, u, L.lnA, L.b, L.theta, L.A);
+        L.lnP = std::log(pv);
+        rx.plog.push_back(L);
+      }
+      finish_plog(rx);
+    } else if (elem) {
       rx.kind = kElementary;
       cti_arrhenius(kf, order, u, rx.lnA, rx.b, rx.theta, rx.A);
     } else if (three) {
@@ -729,6 +811,8 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   int n_pdep = 0, n_eff = 0;
   for (int r = 0; r < R; ++r) {
     const Reaction& rx = *rx_of[r];
+    if (!rx.plog.empty())
+      throw std::runtime_error("PLOG rates run in the block-per-cell layout: " + rx.equation);
     if (static_cast<int>(rx.reactants.size()) > kMaxSideSpecies ||
         static_cast<int>(rx.products.size()) > kMaxSideSpecies)
       throw std::runtime_error("more than 3 species on a reaction side: " + rx.equation);
@@ -995,7 +1079,7 @@ DevMechBlob build_device_tables(const Mechanism& m) {
   return blob;
 }
 
-DevBlkBlob build_block_tables(const Mechanism& m) {
+DevBlkBlob build_block_tables(const Mechanism& m, int work_lds_bytes) {
   const int K = m.K(), R = m.R();
   if (K < 2 || K > 255) throw std::runtime_error("block-per-cell path supports 2..255 species");
   if (R > 65535) throw std::runtime_error("block-per-cell path supports at most 65535 reactions");
@@ -1018,20 +1102,41 @@ DevBlkBlob build_block_tables(const Mechanism& m) {
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return group(a) < group(b); });
     for (int d = 0; d < R; ++d) rx_of[d] = &m.reactions[idx[d]];
   }
-  std::vector<std::map<int, int>> net(R);
+  auto integer_stoich = [](const std::vector<std::pair<int, double>>& side) {
+    for (auto& kv : side)
+      if (kv.second != std::floor(kv.second) || kv.second < 1 || kv.second > 3) return false;
+    return true;
+  };
+  // distinct net coefficients, indexed in the species lists
+  std::vector<double> coefs;
+  auto coef_index = [&](double c) {
+    for (size_t i = 0; i < coefs.size(); ++i)
+      if (coefs[i] == c) return static_cast<uint32_t>(i);
+    if (coefs.size() >= 65535) throw std::runtime_error("too many distinct stoichiometric coefficients");
+    coefs.push_back(c);
+    return static_cast<uint32_t>(coefs.size() - 1);
+  };
+  std::vector<std::map<int, double>> net(R);
+  std::vector<bool> lumped(R, false);
   int n_pdep = 0, n_eff = 0;
   std::vector<int> slot_sp;
   std::vector<uint16_t> slot_begin(R), slot_count(R);
   for (int r = 0; r < R; ++r) {
     const Reaction& rx = *rx_of[r];
-    if (static_cast<int>(rx.reactants.size()) > kBlkSide || static_cast<int>(rx.products.size()) > kBlkSide)
-      throw std::runtime_error("more than 4 species on a reaction side: " + rx.equation);
-    for (auto* side : {&rx.reactants, &rx.products})
-      for (auto& kv : *side)
-        if (kv.second != std::floor(kv.second) || kv.second < 1 || kv.second > 3)
-          throw std::runtime_error("non-integer stoichiometry unsupported: " + rx.equation);
-    for (auto& kv : rx.reactants) net[r][kv.first] -= static_cast<int>(kv.second);
-    for (auto& kv : rx.products) net[r][kv.first] += static_cast<int>(kv.second);
+    if (static_cast<int>(rx.reactants.size()) > kBlkSide)
+      throw std::runtime_error("more than 4 reactant species: " + rx.equation);
+    if (!integer_stoich(rx.reactants)) throw std::runtime_error("non-integer reactant stoichiometry unsupported: " + rx.equation);
+    const bool prod_ok = static_cast<int>(rx.products.size()) <= kBlkSide && integer_stoich(rx.products);
+    if (!prod_ok) {
+      if (rx.reversible)
+        throw std::runtime_error("a reversible reaction needs at most 4 products with integer coefficients: " +
+                                 rx.equation);
+      for (auto& kv : rx.products)
+        if (!(kv.second > 0)) throw std::runtime_error("non-positive product coefficient: " + rx.equation);
+      lumped[r] = true;
+    }
+    for (auto& kv : rx.reactants) net[r][kv.first] -= kv.second;
+    for (auto& kv : rx.products) net[r][kv.first] += kv.second;
     slot_begin[r] = static_cast<uint16_t>(slot_sp.size());
     for (auto& kv : rx.reactants) slot_sp.push_back(kv.first);
     if (rx.reversible)
@@ -1055,13 +1160,42 @@ DevBlkBlob build_block_tables(const Mechanism& m) {
     for (int r = 0; r < R; ++r) {
       auto it = net[r].find(k);
       if (it == net[r].end() || it->second == 0) continue;
-      if (it->second < -128 || it->second > 127) throw std::runtime_error("net stoichiometric coefficient out of range");
-      csr.push_back(static_cast<uint32_t>(r) | (static_cast<uint32_t>(static_cast<uint8_t>(static_cast<int8_t>(it->second))) << 16));
+      csr.push_back(static_cast<uint32_t>(r) | (coef_index(it->second) << 16));
     }
     if (csr.size() > 65535) throw std::runtime_error("too many species-reaction pairs");
   }
   csr_ptr[K] = static_cast<uint16_t>(csr.size());
   L.n_csr = static_cast<int>(csr.size());
+  // products of lumped reactions (their enthalpy and mole change)
+  std::vector<uint16_t> lump_ptr(R + 1, 0);
+  std::vector<uint32_t> lump;
+  for (int r = 0; r < R; ++r) {
+    lump_ptr[r] = static_cast<uint16_t>(lump.size());
+    if (lumped[r])
+      for (auto& kv : rx_of[r]->products) lump.push_back(static_cast<uint32_t>(kv.first) | (coef_index(kv.second) << 16));
+    if (lump.size() > 65535) throw std::runtime_error("too many lumped products");
+  }
+  lump_ptr[R] = static_cast<uint16_t>(lump.size());
+  L.n_lump = static_cast<int>(lump.size());
+  L.n_coef = static_cast<int>(coefs.size());
+  // PLOG rates: device reaction, pressure levels
+  std::vector<uint16_t> plog_rxn, plog_ptr(1, 0);
+  std::vector<double> plog_lv;
+  for (int r = 0; r < R; ++r) {
+    const Reaction& rx = *rx_of[r];
+    if (rx.plog.empty()) continue;
+    plog_rxn.push_back(static_cast<uint16_t>(r));
+    for (const PlogLevel& lv : rx.plog) {
+      plog_lv.push_back(lv.lnP);
+      plog_lv.push_back(lv.lnA);
+      plog_lv.push_back(lv.b);
+      plog_lv.push_back(lv.theta);
+    }
+    if (plog_lv.size() / 4 > 65535) throw std::runtime_error("too many PLOG levels");
+    plog_ptr.push_back(static_cast<uint16_t>(plog_lv.size() / 4));
+  }
+  L.n_plog = static_cast<int>(plog_rxn.size());
+  L.n_plog_lv = static_cast<int>(plog_lv.size() / 4);
   // per-species Jacobian slots (the temperature row's column sums)
   std::vector<uint32_t> col_ptr(K + 1, 0);
   std::vector<uint32_t> col;
@@ -1075,36 +1209,39 @@ DevBlkBlob build_block_tables(const Mechanism& m) {
   col_ptr[K] = static_cast<uint32_t>(col.size());
   L.n_col = static_cast<int>(col.size());
 
+  // tables in staging order; the LDS prefix ends at the last table that fits
   int off = 0;
-  L.off_species = off;
-  off += K * kSpStride * 8;
-  off = align_to(off, 16);
-  L.off_rxn = off;
-  off += R * static_cast<int>(sizeof(BlkRxn));
-  off = align_to(off, 16);
-  L.off_arr = off;
-  off += R * 3 * 8;
-  off = align_to(off, 16);
-  L.off_pdep = off;
-  off += std::max(n_pdep, 1) * static_cast<int>(sizeof(PdepRec));
-  off = align_to(off, 16);
-  L.off_eff_val = off;
-  off += std::max(n_eff, 1) * 8;
-  L.off_eff_sp = off;
-  off += align_to(std::max(n_eff, 1), 16);
-  L.off_csr_ptr = off;
-  off += align_to((K + 1) * 2, 16);
-  L.off_csr = off;
-  off += align_to(std::max(L.n_csr, 1) * 4, 16);
-  L.off_exptab = off;
-  off += 32 * 8;
-  L.lds_bytes = off;
-  L.off_slot_sp = off;
-  off += align_to(std::max(L.n_slots, 1), 16);
-  L.off_col_ptr = off;
-  off += align_to((K + 1) * 4, 16);
-  L.off_col = off;
-  off += align_to(std::max(L.n_col, 1) * 4, 16);
+  std::vector<int> ends;
+  auto place = [&](int32_t& field, int bytes) {
+    field = off;
+    off = align_to(off + std::max(bytes, 1), 16);
+    ends.push_back(off);
+  };
+  place(L.off_exptab, 32 * 8);
+  place(L.off_arr, R * 3 * 8);
+  const int min_staged = off;
+  place(L.off_species, K * kSpStride * 8);
+  place(L.off_rxn, R * static_cast<int>(sizeof(BlkRxn)));
+  place(L.off_pdep, n_pdep * static_cast<int>(sizeof(PdepRec)));
+  place(L.off_eff_val, n_eff * 8);
+  place(L.off_eff_sp, n_eff);
+  place(L.off_csr_ptr, (K + 1) * 2);
+  place(L.off_coef, L.n_coef * 8);
+  place(L.off_csr, L.n_csr * 4);
+  const int budget = kBlkLdsMax - work_lds_bytes;
+  if (min_staged > budget)
+    throw std::runtime_error("the Arrhenius table and the work area exceed the LDS of a CU");
+  L.lds_bytes = min_staged;
+  for (int e : ends)
+    if (e <= budget) L.lds_bytes = std::max(L.lds_bytes, e);
+  place(L.off_slot_sp, L.n_slots);
+  place(L.off_col_ptr, (K + 1) * 4);
+  place(L.off_col, L.n_col * 4);
+  place(L.off_plog_rxn, L.n_plog * 2);
+  place(L.off_plog_ptr, (L.n_plog + 1) * 2);
+  place(L.off_plog_lv, L.n_plog_lv * 4 * 8);
+  place(L.off_lump_ptr, (R + 1) * 2);
+  place(L.off_lump, L.n_lump * 4);
   L.total_bytes = off;
 
   blob.bytes.assign(off, 0);
@@ -1131,13 +1268,15 @@ DevBlkBlob build_block_tables(const Mechanism& m) {
       h.sp[q] = static_cast<uint8_t>(rx.reactants[q].first);
       h.nu[q] = static_cast<uint8_t>(rx.reactants[q].second);
     }
-    for (size_t q = 0; q < rx.products.size(); ++q) {
-      h.sp[kBlkSide + q] = static_cast<uint8_t>(rx.products[q].first);
-      h.nu[kBlkSide + q] = static_cast<uint8_t>(rx.products[q].second);
-    }
+    if (!lumped[r])
+      for (size_t q = 0; q < rx.products.size(); ++q) {
+        h.sp[kBlkSide + q] = static_cast<uint8_t>(rx.products[q].first);
+        h.nu[kBlkSide + q] = static_cast<uint8_t>(rx.products[q].second);
+      }
     h.kind = static_cast<uint8_t>(rx.kind);
-    const bool constk = (rx.b == 0.0 && rx.theta == 0.0);
-    h.flags = (rx.reversible ? 1 : 0) | (constk ? 2 : 0) | (rx.troe_has_t2 ? 4 : 0);
+    // a PLOG rate's Arrhenius parameters change with the cell's pressure: never a constant
+    const bool constk = rx.plog.empty() && (rx.b == 0.0 && rx.theta == 0.0);
+    h.flags = (rx.reversible ? 1 : 0) | (constk ? 2 : 0) | (rx.troe_has_t2 ? 4 : 0) | (lumped[r] ? kBlkLumped : 0);
     h.pidx = 0xFFFF;
     h.slot = slot_begin[r];
     h.nslot = slot_count[r];
@@ -1166,12 +1305,18 @@ DevBlkBlob build_block_tables(const Mechanism& m) {
     }
     rxn[r] = h;
   }
-  std::memcpy(B + L.off_csr_ptr, csr_ptr.data(), csr_ptr.size() * 2);
-  if (!csr.empty()) std::memcpy(B + L.off_csr, csr.data(), csr.size() * 4);
   std::memcpy(B + L.off_exptab, kExp2Tab32, sizeof(kExp2Tab32));
+  std::memcpy(B + L.off_csr_ptr, csr_ptr.data(), csr_ptr.size() * 2);
+  if (!coefs.empty()) std::memcpy(B + L.off_coef, coefs.data(), coefs.size() * 8);
+  if (!csr.empty()) std::memcpy(B + L.off_csr, csr.data(), csr.size() * 4);
   for (int q = 0; q < L.n_slots; ++q) B[L.off_slot_sp + q] = static_cast<uint8_t>(slot_sp[q]);
   std::memcpy(B + L.off_col_ptr, col_ptr.data(), col_ptr.size() * 4);
   if (!col.empty()) std::memcpy(B + L.off_col, col.data(), col.size() * 4);
+  if (!plog_rxn.empty()) std::memcpy(B + L.off_plog_rxn, plog_rxn.data(), plog_rxn.size() * 2);
+  std::memcpy(B + L.off_plog_ptr, plog_ptr.data(), plog_ptr.size() * 2);
+  if (!plog_lv.empty()) std::memcpy(B + L.off_plog_lv, plog_lv.data(), plog_lv.size() * 8);
+  std::memcpy(B + L.off_lump_ptr, lump_ptr.data(), lump_ptr.size() * 2);
+  if (!lump.empty()) std::memcpy(B + L.off_lump, lump.data(), lump.size() * 4);
   return blob;
 }
 

@@ -23,6 +23,11 @@ constexpr int kMaxSideSpecies = 3;   // species per reaction side on the device 
 
 enum RxnKind : int { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3 };
 
+// One pressure level of a pressure-dependent Arrhenius rate (PLOG, Cantera's PlogRate).
+struct PlogLevel {
+  double lnP = 0, lnA = 0, b = 0, theta = 0, A = 0;   // ln of the pressure in Pa; SI-kmol Arrhenius
+};
+
 struct Reaction {
   std::string equation;
   int kind = kElementary;
@@ -34,6 +39,9 @@ struct Reaction {
   double eff_base = 1.0;
   double troe[4] = {0, 0, 0, 0};                           // a, T3, T1, T2
   bool troe_has_t2 = false;
+  // PLOG (kind stays kElementary): ln k interpolated linearly in ln P between the levels that
+  // bracket the cell's pressure, the end level's rate outside them; levels ascending in P.
+  std::vector<PlogLevel> plog;
 };
 
 struct Mechanism {
@@ -153,32 +161,48 @@ struct alignas(8) BlkRxn {      // 24 bytes
 static_assert(sizeof(BlkRxn) == 24, "BlkRxn layout");
 
 struct BlkLayout {
-  int32_t K, R, n_pdep, n_eff, n_slots, n_csr, n_col;
-  int32_t lds_bytes;            // bytes [0, lds_bytes) are staged to LDS
-  // LDS part
+  int32_t K, R, n_pdep, n_eff, n_slots, n_csr, n_col, n_coef, n_plog, n_plog_lv, n_lump;
+  // Bytes [0, lds_bytes) are staged to LDS: a prefix of the tables below, in this order, as many
+  // as fit beside the kernel's work area (always through the Arrhenius table, which PLOG rewrites
+  // per cell); the others are read from global memory through the cache.
+  int32_t lds_bytes;
+  int32_t off_exptab;           // double[32]: 2^(j/32)
+  int32_t off_arr;              // double[R][3]: lnA (or A), b, theta; PLOG reactions: per cell
   int32_t off_species;          // double[K][kSpStride]
   int32_t off_rxn;              // BlkRxn[R]
-  int32_t off_arr;              // double[R][3]: lnA (or A), b, theta
   int32_t off_pdep;             // PdepRec[n_pdep]
   int32_t off_eff_val;          // double[n_eff]
   int32_t off_eff_sp;           // uint8[n_eff]
   int32_t off_csr_ptr;          // uint16[K+1]: reactions of species k
-  int32_t off_csr;              // uint32[n_csr]: r | (net coefficient, int8) << 16, ascending r
-  int32_t off_exptab;           // double[32]: 2^(j/32)
-  // global part
+  int32_t off_coef;             // double[n_coef]: the distinct net stoichiometric coefficients
+  int32_t off_csr;              // uint32[n_csr]: r | coefficient index << 16, ascending r
+  // global only
   int32_t off_slot_sp;          // uint8[n_slots]: species of each d(phi)/dC slot
   int32_t off_col_ptr;          // uint32[K+1]: Jacobian slots whose species is k
   int32_t off_col;              // uint32[n_col]: slot | reaction << 16
+  int32_t off_plog_rxn;         // uint16[n_plog]: device reaction of each PLOG rate
+  int32_t off_plog_ptr;         // uint16[n_plog+1]: its pressure levels
+  int32_t off_plog_lv;          // double[n_plog_lv][4]: ln P (Pa), lnA, b, theta, ascending in P
+  int32_t off_lump_ptr;         // uint16[R+1]: products of lumped reactions (flag 16)
+  int32_t off_lump;             // uint32[n_lump]: species | coefficient index << 16
   int32_t total_bytes;
 };
+
+constexpr int kBlkLdsMax = 160 * 1024;   // LDS per CU (gfx950)
+constexpr uint8_t kBlkLumped = 16;       // BlkRxn::flags: products in the lump list, not the slots
 
 struct DevBlkBlob {
   BlkLayout lay{};
   std::vector<uint8_t> bytes;
 };
 
-// Throws std::runtime_error when the mechanism does not fit (more than 255 species, more than
-// 4 species on a reaction side, non-integer stoichiometry, more than 65535 reactions or slots).
-DevBlkBlob build_block_tables(const Mechanism& m);
+// work_lds_bytes: the kernel's LDS work area beside the tables, which bounds the staged prefix.
+// Throws std::runtime_error when the mechanism does not fit: more than 255 species, more than 4
+// reactant species or 4 products of a reversible reaction, non-integer reactant or reversible
+// product stoichiometry, more than 65535 reactions, slots or coefficients, or an LDS work area
+// with no room for the Arrhenius table. Irreversible reactions may have any number of products
+// with any positive coefficient ("lumped": products enter the production rates and the reaction
+// enthalpy only).
+DevBlkBlob build_block_tables(const Mechanism& m, int work_lds_bytes);
 
 }  // namespace hpo

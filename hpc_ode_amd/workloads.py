@@ -95,3 +95,21 @@ def dodecane_air(species, W, n, seed=0, T_range=(1000.0, 1600.0), P_atm_range=(1
     X /= X.sum(axis=1, keepdims=True)
     y = _to_state(X, W, T)
     return y, P, np.full(n, dt)
+
+
+def kerosene_air(species, W, n, seed=0, T_range=(1000.0, 2100.0), phi_range=(0.5, 3.5), P_atm=1.0, dt=1e-4):
+    """n-dodecane/air cells for ESTiMatE-Mech (214 species), the states the reference's
+    ref_data/generate.py tabulates for that mechanism: fuel N-C12H26 in air (O2:N2 = 1:3.76),
+    X_fuel : X_O2 = phi : 18.5, T0 ~ U(T_range), phi ~ U(phi_range), at P_atm atmospheres."""
+    rng = np.random.default_rng(seed)
+    K = len(species)
+    W = np.asarray(W)
+    T = rng.uniform(*T_range, n)
+    phi = rng.uniform(*phi_range, n)
+    X = np.zeros((n, K))
+    X[:, species.index("N-C12H26")] = phi
+    X[:, species.index("O2")] = 18.5
+    X[:, species.index("N2")] = 18.5 * 3.76
+    X /= X.sum(axis=1, keepdims=True)
+    y = _to_state(X, W, T)
+    return y, np.full(n, P_atm * ATM), np.full(n, dt)

@@ -30,6 +30,7 @@ class OrMech(C.Structure):
         ("reac_sp", _ip), ("prod_sp", _ip), ("reac_nu", _dp), ("prod_nu", _dp),
         ("arr", _dp), ("troe", _dp), ("eff_ptr", _ip), ("eff_sp", _ip),
         ("eff_val", _dp), ("eff_base", _dp), ("conv", C.c_int),
+        ("plog_ptr", _ip), ("plog", _dp),
     ]
 
 
@@ -122,14 +123,20 @@ class Oracle:
             troe=np.ascontiguousarray(t.troe), eff_ptr=t.eff_ptr,
             eff_sp=t.eff_sp if len(t.eff_sp) else np.zeros(1, np.int32),
             eff_val=t.eff_val if len(t.eff_val) else np.zeros(1),
-            eff_base=np.ascontiguousarray(t.eff_base))
+            eff_base=np.ascontiguousarray(t.eff_base),
+            plog_ptr=np.ascontiguousarray(t.plog_ptr),
+            plog=np.ascontiguousarray(t.plog.ravel()) if len(t.plog) else np.zeros(4))
+        if reactor == "CONV" and len(t.plog):
+            # the pressure of a constant-volume cell moves with its state, and the Jacobian would
+            # need d k / d P terms for the PLOG rates
+            raise ValueError("PLOG rates are supported at constant pressure only")
         self._keep = keep
         s = OrMech()
         s.K, s.R = t.K, t.R
         s.conv = REACTORS[reactor]
-        for name in ("W", "tmid", "nasa", "reac_nu", "prod_nu", "arr", "troe", "eff_val", "eff_base"):
+        for name in ("W", "tmid", "nasa", "reac_nu", "prod_nu", "arr", "troe", "eff_val", "eff_base", "plog"):
             setattr(s, name, _p(keep[name]))
-        for name in ("kind", "rev", "nreac", "nprod", "reac_sp", "prod_sp", "eff_ptr", "eff_sp"):
+        for name in ("kind", "rev", "nreac", "nprod", "reac_sp", "prod_sp", "eff_ptr", "eff_sp", "plog_ptr"):
             setattr(s, name, _p(keep[name], _ip))
         self.s = s
         self.K = t.K

@@ -53,6 +53,9 @@ class Reaction:
     default_eff: float = 1.0
     troe: Tuple[float, float, float, float] = (0.0, 0.0, 0.0, 0.0)
     troe_has_t2: bool = False
+    # pressure-dependent Arrhenius (PLOG): (P in Pa, A, b, theta) per level, ascending in P;
+    # empty for every other reaction. The reaction is elementary otherwise.
+    plog: List[Tuple[float, float, float, float]] = dataclasses.field(default_factory=list)
 
 
 @dataclasses.dataclass
@@ -91,6 +94,21 @@ def _ea_factor(units: dict) -> float:
 
 
 _TERM = re.compile(r"^\s*(?:(\d+(?:\.\d+)?)\s+)?(\S+)\s*$")
+
+
+PRESSURE_UNITS = {"Pa": 1.0, "kPa": 1e3, "MPa": 1e6, "bar": 1e5, "atm": P_ATM, "torr": P_ATM / 760.0}
+
+
+def _plog_levels(levels):
+    """Sorted (P, A, b, theta) levels; a pressure given twice (a sum of rates at one level) and a
+    non-positive A are refused."""
+    levels = sorted(levels)
+    ps = [lv[0] for lv in levels]
+    if len(set(ps)) != len(ps):
+        raise ValueError("PLOG with several rate expressions at one pressure unsupported")
+    if any(lv[1] <= 0 for lv in levels):
+        raise ValueError("PLOG with a non-positive pre-exponential factor unsupported")
+    return levels
 
 
 def _parse_side(side: str) -> Tuple[List[Tuple[str, float]], str | None, bool]:
@@ -210,6 +228,12 @@ def load_xml(path: str) -> Mechanism:
         typ = x.get("type", "elementary")
         if typ == "elementary":
             rx = Reaction(eq, RXN_ELEMENTARY, rev, reac, prod, A, b, th)
+        elif typ == "plog":
+            levels = []
+            for a in rc.findall("Arrhenius"):
+                pe = a.find("P")
+                levels.append((float(pe.text) * PRESSURE_UNITS[pe.get("units", "Pa")], *arr(a)))
+            rx = Reaction(eq, RXN_ELEMENTARY, rev, reac, prod, A, b, th, plog=_plog_levels(levels))
         elif typ == "threeBody":
             rx = Reaction(eq, RXN_THREE_BODY, rev, reac, prod, A, b, th, efficiencies=effs, default_eff=deff)
         elif typ == "falloff":
@@ -318,6 +342,16 @@ def load_yaml(path: str) -> Mechanism:
         elif typ == "elementary":
             A, b, th = arr(rd["rate-constant"], order)
             rx = Reaction(eq, RXN_ELEMENTARY, rev, reac, prod, A, b, th)
+        elif typ == "pressure-dependent-Arrhenius":
+            levels = []
+            for rc in rd["rate-constants"]:
+                pv = rc["P"]
+                if isinstance(pv, str):
+                    v, u = pv.split()
+                    pv = float(v) * PRESSURE_UNITS[u]
+                levels.append((float(pv), *arr(rc, order)))
+            levels = _plog_levels(levels)
+            rx = Reaction(eq, RXN_ELEMENTARY, rev, reac, prod, *levels[0][1:], plog=levels)
         else:
             raise ValueError(f"reaction type {typ} unsupported")
         rxns.append(rx)
@@ -327,7 +361,7 @@ def load_yaml(path: str) -> Mechanism:
 # ----------------------------------------------------------------------------
 # Flat tables for the C oracle (oracle.h: struct or_mech)
 # ----------------------------------------------------------------------------
-MAX_SLOTS = 4   # reactant / product species per reaction side
+MAX_SLOTS = 6   # reactant / product species per reaction side
 
 
 @dataclasses.dataclass
@@ -351,6 +385,8 @@ class Tables:
     eff_sp: np.ndarray        # [E] int32
     eff_val: np.ndarray       # [E] float64  (efficiency - 1)
     eff_base: np.ndarray      # [R] float64 default efficiency
+    plog_ptr: np.ndarray      # [R+1] int32: levels of reaction r are plog[plog_ptr[r]:plog_ptr[r+1]]
+    plog: np.ndarray          # [L, 4]: ln P (Pa), lnA, b, theta
 
 
 def tables(m: Mechanism) -> Tables:
@@ -370,6 +406,8 @@ def tables(m: Mechanism) -> Tables:
     eff_sp: List[int] = []
     eff_val: List[float] = []
     eff_base = np.ones(R)
+    plog_ptr = [0]
+    plog: List[Tuple[float, float, float, float]] = []
     for r, rx in enumerate(m.reactions):
         kind[r] = rx.kind
         rev[r] = int(rx.reversible)
@@ -392,8 +430,12 @@ def tables(m: Mechanism) -> Tables:
                 eff_sp.append(k)
                 eff_val.append(rx.efficiencies[k] - rx.default_eff)
         eff_ptr.append(len(eff_sp))
+        for P, A, b, th in rx.plog:
+            plog.append((np.log(P), np.log(A), b, th))
+        plog_ptr.append(len(plog))
     nasa = np.concatenate([m.nasa_lo, m.nasa_hi], axis=1)
     return Tables(K, R, m.W.copy(), m.tmid.copy(), np.ascontiguousarray(nasa), kind, rev,
                   nreac, nprod, reac_sp, reac_nu, prod_sp, prod_nu, arr, troe,
                   np.asarray(eff_ptr, np.int32), np.asarray(eff_sp, np.int32),
-                  np.asarray(eff_val, np.float64), eff_base)
+                  np.asarray(eff_val, np.float64), eff_base,
+                  np.asarray(plog_ptr, np.int32), np.asarray(plog, np.float64).reshape(-1, 4))

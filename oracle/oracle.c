@@ -41,6 +41,37 @@ static double karr(double lnA, double b, double th, double A, double lnT, double
   return exp(lnA + b * lnT - th / T);
 }
 
+/* Pressure-dependent Arrhenius ("PLOG", Cantera's PlogRate): ln k is interpolated linearly in
+ * ln P between the two levels that bracket the pressure, each level an Arrhenius expression;
+ * below the first level or above the last, that level's rate. Also returns d ln k / dT at fixed
+ * P. lv: n levels of (ln P, lnA, b, theta), ascending in P. */
+static void plog_rate(const double *lv, int n, double lnP, double lnT, double T, double *k, double *dlnk) {
+  int i = 0;
+  double w = 0.0;
+  if (lnP <= lv[0]) {
+    i = 0;
+    w = 0.0;
+  } else if (lnP >= lv[4 * (n - 1)]) {
+    i = n - 1;
+    w = 0.0;
+  } else {
+    while (i + 1 < n && lv[4 * (i + 1)] <= lnP) ++i;
+    w = (lnP - lv[4 * i]) / (lv[4 * (i + 1)] - lv[4 * i]);
+  }
+  const double *a = lv + 4 * i;
+  double lk = a[1] + a[2] * lnT - a[3] / T;
+  double dl = (a[2] + a[3] / T) / T;
+  if (w != 0.0) {
+    const double *c = a + 4;
+    const double lk2 = c[1] + c[2] * lnT - c[3] / T;
+    const double dl2 = (c[2] + c[3] / T) / T;
+    lk += w * (lk2 - lk);
+    dl += w * (dl2 - dl);
+  }
+  *k = exp(lk);
+  *dlnk = dl;
+}
+
 static double ipow(double c, double nu) {
   if (nu == 1.0) return c;
   if (nu == 2.0) return c * c;
@@ -112,8 +143,13 @@ static void eval_rxn(const or_mech *m, int r, const state_t *s, rxn_t *o) {
   const int nr = m->nreac[r], np = m->nprod[r];
   const int *rs = m->reac_sp + OR_MAX_SLOTS * r, *ps = m->prod_sp + OR_MAX_SLOTS * r;
   const double *rn = m->reac_nu + OR_MAX_SLOTS * r, *pn = m->prod_nu + OR_MAX_SLOTS * r;
-  o->kf = karr(a[0], a[1], a[2], a[3], lnT, T);
-  o->dlnkf = (a[1] + a[2] / T) / T;
+  if (m->plog_ptr && m->plog_ptr[r + 1] > m->plog_ptr[r]) {
+    plog_rate(m->plog + 4 * m->plog_ptr[r], m->plog_ptr[r + 1] - m->plog_ptr[r], log(s->P), lnT, T, &o->kf,
+              &o->dlnkf);
+  } else {
+    o->kf = karr(a[0], a[1], a[2], a[3], lnT, T);
+    o->dlnkf = (a[1] + a[2] / T) / T;
+  }
   double fwd = o->kf;
   for (int i = 0; i < nr; ++i) fwd *= ipow(s->C[rs[i]], rn[i]);
   o->fwd = fwd;

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define OR_MAX_SLOTS 4
+#define OR_MAX_SLOTS 6
 
 /* Flat mechanism tables (filled by oracle/mech.py). */
 typedef struct {
@@ -50,6 +50,9 @@ typedef struct {
   int conv;              /* reactor: 0 constant pressure (the P argument of the functions below is
                             the pressure, Pa); 1 constant volume (the P argument is the density,
                             kg/m^3, held through the integration) */
+  const int *plog_ptr;   /* [R+1] or NULL: pressure levels of reaction r are plog[plog_ptr[r] ..
+                            plog_ptr[r+1]) (none: a plain Arrhenius rate) */
+  const double *plog;    /* [L][4]: ln P (Pa), lnA, b, theta of each level, ascending in P */
 } or_mech;
 
 typedef struct {

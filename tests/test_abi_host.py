@@ -1,5 +1,6 @@
 """Host-side checks of the C ABI (no GPU calls): the library loads, exports every function
 include/hpc_ode.h declares, parses mechanisms like the oracle does, and reports errors."""
+import os
 import ctypes as C
 import subprocess
 
@@ -26,11 +27,13 @@ def test_no_oracle_in_product_library():
     assert "liboracle" not in ldd and "pyjac" not in ldd
 
 
-@pytest.mark.parametrize("which", ["gri", "h2", "h2li"])
+@pytest.mark.parametrize("which", ["gri", "h2", "h2li", "est"])
 def test_cpp_reader_matches_oracle_reader(which, gri_path, h2_path, h2li_path):
-    """The product's C++ readers (YAML; CTML for h2_li_2004.xml) and the oracle's readers
-    (PyYAML; xml.etree) give identical tables."""
-    path = {"gri": gri_path, "h2": h2_path, "h2li": h2li_path}[which]
+    """The product's C++ readers (YAML; CTML for h2_li_2004.xml and ESTiMatE-Mech, whose PLOG
+    reactions report their lowest-pressure level here) and the oracle's readers (PyYAML;
+    xml.etree) give identical tables."""
+    est = os.path.join(os.path.dirname(h2_path), "ESTiMatE-Mech_04042022.xml")
+    path = {"gri": gri_path, "h2": h2_path, "h2li": h2li_path, "est": est}[which]
     m = H.Mechanism(path)
     o = O.Oracle(path)
     assert m.species == o.mech.species

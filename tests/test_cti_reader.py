@@ -91,7 +91,8 @@ three_body_reaction('2 H + M <=> H2 + M', [1.0e+18, -1.0, 0.0],
      "NASA7"),
     ("units(length='cm', quantity='mol', act_energy='cal/mol')\nideal_gas(name='x', species='H', reactions='all')\n"
      "species(name='H', atoms='H:1', thermo=(NASA([200.0, 1000.0], [2.5,0,0,0,0,25473.0,-0.44]),))\n"
-     "pdep_arrhenius('H + H <=> H', [(1.0, 'atm'), 1.0, 0.0, 0.0])\n", "pdep_arrhenius unsupported"),
+     "chebyshev_reaction('H + H <=> H', Tmin=300.0, Tmax=2000.0, Pmin=(0.1, 'atm'), Pmax=(10.0, 'atm'), "
+     "coeffs=[[1.0]])\n", "chebyshev_reaction unsupported"),
     ("ideal_gas(name='x', species='H' reactions='all')\n", "expected ',' or ')'"),
     ("ideal_gas(name='x', species='H', reactions='all'\n", "expected"),
 ])
@@ -100,3 +101,49 @@ def test_cti_errors_are_reported(tmp_path, text, err):
     f.write_text(text)
     with pytest.raises(H.HpoError, match=re.escape(err)):
         H.Mechanism(str(f))
+
+
+def test_cti_and_yaml_plog_read_alike(tmp_path):
+    """A pressure-dependent Arrhenius reaction (CTI pdep_arrhenius, YAML
+    pressure-dependent-Arrhenius) reads the same from both forms: the same kind, and the
+    lowest-pressure level's SI-kmol parameters in the plain Arrhenius columns (the levels drive
+    the kernels; tests/test_gpu_estimate.py checks them on ESTiMatE-Mech's PLOG rates)."""
+    thermo = ("thermo=(NASA([200.0, 1000.0], [2.5, 0.0, 0.0, 0.0, 0.0, 25473.0, -0.44]), "
+              "NASA([1000.0, 3500.0], [2.5, 0.0, 0.0, 0.0, 0.0, 25473.0, -0.44]))")
+    cti = tmp_path / "p.cti"
+    cti.write_text(f"""units(length='cm', time='s', quantity='mol', act_energy='cal/mol')
+ideal_gas(name='p', elements='H', species='H H2', reactions='all')
+species(name='H', atoms='H:1', {thermo})
+species(name='H2', atoms='H:2', {thermo})
+pdep_arrhenius('H + H <=> H2', [(10.0, 'atm'), 3.0e+13, 0.2, 1200.0], [(0.1, 'atm'), 1.0e+13, 0.0, 1000.0],
+               [(1.0, 'atm'), 2.0e+13, 0.1, 1100.0])
+""")
+    nasa = "[2.5, 0.0, 0.0, 0.0, 0.0, 25473.0, -0.44]"
+    yml = tmp_path / "p.yaml"
+    yml.write_text(f"""units: {{length: cm, time: s, quantity: mol, activation-energy: cal/mol}}
+phases:
+- name: p
+  thermo: ideal-gas
+  elements: [H]
+  species: [H, H2]
+species:
+- name: H
+  composition: {{H: 1}}
+  thermo: {{model: NASA7, temperature-ranges: [200.0, 1000.0, 3500.0], data: [{nasa}, {nasa}]}}
+- name: H2
+  composition: {{H: 2}}
+  thermo: {{model: NASA7, temperature-ranges: [200.0, 1000.0, 3500.0], data: [{nasa}, {nasa}]}}
+reactions:
+- equation: H + H <=> H2
+  type: pressure-dependent-Arrhenius
+  rate-constants:
+  - {{P: 0.1 atm, A: 1.0e+13, b: 0.0, Ea: 1000.0}}
+  - {{P: 1.0 atm, A: 2.0e+13, b: 0.1, Ea: 1100.0}}
+  - {{P: 10.0 atm, A: 3.0e+13, b: 0.2, Ea: 1200.0}}
+""")
+    c, y = H.Mechanism(str(cti)), H.Mechanism(str(yml))
+    tc, ty = c.tables(), y.tables()
+    assert list(tc["kind"]) == list(ty["kind"]) == [0]
+    assert np.array_equal(tc["arr"], ty["arr"])
+    assert tc["arr"][0, 3] == pytest.approx(1.0e13 * 1e-3, rel=1e-15)
+    assert tc["arr"][0, 2] == pytest.approx(1000.0 * 4184.0 / 8314.4621, rel=1e-15)
