@@ -36,13 +36,13 @@ class Opts(C.Structure):
                 ("lu_precision", C.c_int32), ("refine_sweeps", C.c_int32),
                 ("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("sort_cells", C.c_int32), ("lu_layout", C.c_int32), ("cell_layout", C.c_int32),
-                ("reactor", C.c_int32)]
+                ("reactor", C.c_int32), ("fp64_fallback", C.c_int32)]
 
 
 class Timing(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("ncells", C.c_int64),
                 ("grid", C.c_int32), ("block", C.c_int32), ("lds_bytes", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("fallback_cells", C.c_int32)]
 
 
 TUNE_MAX = 16
@@ -163,7 +163,8 @@ def _reactor(r):
 
 
 def opts(rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0, refine_sweeps=1,
-         waves_per_block=0, blocks_per_cu=0, sort_cells=0, lu_layout=0, cell_layout=0, reactor=0) -> Opts:
+         waves_per_block=0, blocks_per_cu=0, sort_cells=0, lu_layout=0, cell_layout=0, reactor=0,
+         fp64_fallback=1) -> Opts:
     o = Opts()
     lib().hpo_opts_default(C.byref(o))
     o.rtol, o.atol, o.maxsteps = rtol, atol, maxsteps
@@ -172,6 +173,7 @@ def opts(rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0, refine_sweeps=1,
     o.lu_layout = lu_layout
     o.cell_layout = cell_layout
     o.reactor = _reactor(reactor)
+    o.fp64_fallback = fp64_fallback
     return o
 
 
@@ -263,7 +265,7 @@ class Mechanism:
     # --- integration ------------------------------------------------------
     def integrate(self, y, P, dt, rtol=1e-6, atol=1e-10, maxsteps=10000, lu_precision=0,
                   refine_sweeps=1, waves_per_block=0, blocks_per_cu=0, sort_cells=0,
-                  return_timing=False, lu_layout=0, cell_layout=0, reactor=0):
+                  return_timing=False, lu_layout=0, cell_layout=0, reactor=0, fp64_fallback=1):
         """Integrates every row of y ([ncells, n_state]) from t=0 to dt on the GPU. reactor
         "CONP": P is each cell's pressure; "CONV": P is each cell's density (see density()).
         Returns (y_out, status, stats[, timing])."""
@@ -272,7 +274,7 @@ class Mechanism:
         b = Batch(self, n)
         b.put(y, P, dt)
         t = b.integrate(n, opts(rtol, atol, maxsteps, lu_precision, refine_sweeps, waves_per_block,
-                                blocks_per_cu, sort_cells, lu_layout, cell_layout, reactor))
+                                blocks_per_cu, sort_cells, lu_layout, cell_layout, reactor, fp64_fallback))
         yo, st, stats = b.get(n)
         return (yo, st, stats, t) if return_timing else (yo, st, stats)
 

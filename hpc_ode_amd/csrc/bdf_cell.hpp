@@ -600,8 +600,11 @@ __device__ __forceinline__ void integrate_cell(const BdfArgs& A, Lay& Ly, Scal& 
       }
     }
   }
-  if (act) A.y_out[cell * A.ld + lane] = yv;
+  // with the FP64 fallback armed, a failed FP32 cell keeps its input row for the FP64 launch
+  const bool keep_row = A.fallback == 1 && hpo_fallback_status(status);
+  if (act && !keep_row) A.y_out[cell * A.ld + lane] = yv;
   if (lane == 0) {
+    if (keep_row) atomicAdd(A.queue + 1, 1u);
     A.status[cell] = status;
     if (A.stats) {
       int* st = A.stats + cell * HPO_NSTATS;

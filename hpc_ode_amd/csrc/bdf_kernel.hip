@@ -159,15 +159,15 @@ struct WaveLayout {
 // H2/air cells (second substep, 17 steps per cell): 28.8 ms FP32 / 29.8 ms FP64 at two waves,
 // 27.1 / 29.7 ms at three (168 registers), with identical results.
 //
-// An earlier three-wave FP64 build returned convergence failures for every cell once a
-// never-taken branch was added to the cell loop. That build wrote the LU's permutation with an
-// inline-asm v_writelane, which writes its lane whatever EXEC holds; as an asm operand the
-// register allocator could give its register to a value still live in lanes outside EXEC,
-// and the tighter three-wave register budget makes such sharing likelier. The write is now a
-// compare and select (wli), and the same three-wave build with a never-taken branch of the
-// same form (a per-workgroup exit test before each queue pull) matches the oracle.
+// Two is the default. Three-wave FP64 builds of the small kernels have twice returned a Newton
+// convergence failure for every cell after a change to the cell loop that leaves the work of a
+// cell unchanged: once with a never-taken per-workgroup exit test before each queue pull, and
+// once (the constant-volume 9-variable kernel) with the FP64-fallback queue of this build. The
+// first was thought to come from an inline-asm v_writelane (now a compare and select, wli); the
+// second shows that was not the whole cause, and none has been shown. The two-wave builds of
+// the same sources match the oracle.
 #ifndef HPO_WAVES_PER_EU_SMALL
-#define HPO_WAVES_PER_EU_SMALL 3
+#define HPO_WAVES_PER_EU_SMALL 2
 #endif
 template <int NS, bool LU32>
 constexpr int bdf_waves_per_eu() {
@@ -199,12 +199,34 @@ __global__ __launch_bounds__(256) __attribute__((
   static_assert((jx_cols<NS>() - 1) * NS + 64 <= 64 * 64, "padded J x columns leave the Jacobian block");
   for (int i = NS * NS + lane; i < (jx_cols<NS>() - 1) * NS + 64; i += 64) g.J[i] = 0.0;
   wave_sync();
+  // FP64 launches after an FP32 one with the fallback armed take only the cells that failed
+  // there, 64 cells per queue pull (their statuses read one per lane), from their input rows
+  const bool fb = !LU32 && A.fallback == 2;
+  if (fb && __builtin_amdgcn_readfirstlane(static_cast<int>(*reinterpret_cast<volatile uint32_t*>(A.queue + 1))) == 0)
+    return;
+  unsigned long long todo = 0ull;
+  unsigned int c0 = 0;
   for (;;) {
-    unsigned int c = 0;
-    if (lane == 0) c = atomicAdd(A.queue, 1u);
-    c = __builtin_amdgcn_readfirstlane(c);
-    if (static_cast<int64_t>(c) >= A.ncells) break;
-    int64_t cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
+    int64_t cell;
+    if (fb) {
+      while (todo == 0ull) {
+        unsigned int c = 0;
+        if (lane == 0) c = atomicAdd(A.queue + 2, 64u);
+        c0 = __builtin_amdgcn_readfirstlane(c);
+        if (static_cast<int64_t>(c0) >= A.ncells) break;
+        const int64_t mine = static_cast<int64_t>(c0) + lane;
+        todo = __ballot(mine < A.ncells && hpo_fallback_status(A.status[mine]));
+      }
+      if (todo == 0ull) break;
+      cell = static_cast<int64_t>(c0) + (__ffsll(todo) - 1);
+      todo &= todo - 1;
+    } else {
+      unsigned int c = 0;
+      if (lane == 0) c = atomicAdd(A.queue, 1u);
+      c = __builtin_amdgcn_readfirstlane(c);
+      if (static_cast<int64_t>(c) >= A.ncells) break;
+      cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
+    }
     WaveLayout<NS, LU32, INV> Ly(t, w, g, lane);
     integrate_cell<NS, LU32, PROF>(A, Ly, S, cell);
   }
@@ -328,10 +350,10 @@ const void* bdf_kernel_fn(int ns, int variant, int prof, int conv) {
 }
 
 namespace dev {
-// Zeroes a launch's work-queue counter on the launch's stream. A kernel rather than a memset, so
+// Zeroes a launch's work-queue counters (four words) on the launch's stream. A kernel rather than a memset, so
 // that a caller capturing the launch into a HIP graph records two kernel nodes.
 __global__ __launch_bounds__(64) void reset_counter(uint32_t* q) {
-  if (threadIdx.x == 0) q[0] = 0u;
+  if (threadIdx.x < 4) q[threadIdx.x] = 0u;   // the queue, the FP64-fallback count and queue
 }
 }  // namespace dev
 

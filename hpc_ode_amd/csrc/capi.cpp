@@ -121,6 +121,7 @@ void check_opts(const hpo_opts& o) {
   if (o.lu_layout != 0 && o.lu_layout != 1) throw std::invalid_argument("lu_layout must be 0 or 1");
   if (o.cell_layout < 0 || o.cell_layout > 2) throw std::invalid_argument("cell_layout must be 0, 1 or 2");
   if (o.reactor < 0 || o.reactor > 1) throw std::invalid_argument("reactor must be 0 (constant pressure) or 1 (constant volume)");
+  if (o.fp64_fallback != 0 && o.fp64_fallback != 1) throw std::invalid_argument("fp64_fallback must be 0 or 1");
 }
 
 bool use_block_layout(const hpo_mech* m, const hpo_opts& o) {
@@ -138,6 +139,31 @@ bool use_block_layout(const hpo_mech* m, const hpo_opts& o) {
   if (wave_ok) return false;
   if (block_ok) return true;
   throw std::invalid_argument("no kernel instantiated for a state of " + std::to_string(ns) + " variables");
+}
+
+// The FP64 launch that follows an FP32 one whose cells had the fallback armed (A.fallback was 1):
+// the same arguments in fallback mode 2, on a grid that fits the FP64 kernel's occupancy, the
+// share of the device and the scratch the FP32 launch used (the FP64 kernel never holds more
+// waves per CU than the FP32 one, so the grid is rarely cut by it).
+static void launch_fp64_fallback(const hpo_mech* m, bool block_layout, hpo::BdfArgs A, int wpb, int block,
+                                 size_t lds, int share_pct, int64_t scratch_doubles, hipStream_t st) {
+  const int ns = m->host.K();
+  const bool prof = A.prof != nullptr;
+  const void* fn = block_layout ? hpo::bdf_block_kernel_fn(ns, 0, prof) : hpo::bdf_kernel_fn(ns, 0, prof, A.reactor);
+  int occ = 0;
+  HPO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
+  if (occ < 1) throw std::invalid_argument("the FP64 kernel does not fit on a CU with this configuration");
+  int64_t grid = static_cast<int64_t>(m->n_cu) * occ;
+  if (share_pct < 100) grid = (grid * std::max(share_pct, 1) + 99) / 100;
+  grid = std::min<int64_t>(grid, (A.ncells + wpb - 1) / wpb);
+  grid = std::min<int64_t>(grid, scratch_doubles / (static_cast<int64_t>(wpb) * A.scratch_stride));
+  grid = std::max<int64_t>(grid, 1);
+  A.fallback = 2;
+  A.order = nullptr;
+  if (block_layout)
+    HPO_HIP(hpo::launch_bdf_block(ns, A, 0, static_cast<int>(grid), lds, st));
+  else
+    HPO_HIP(hpo::launch_bdf(ns, A, 0, static_cast<int>(grid), block, lds, st));
 }
 
 // Launches the persistent BDF kernel on arrays already resident on the device.
@@ -222,6 +248,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   A.refine_sweeps = o.refine_sweeps;
   A.prof = reinterpret_cast<int64_t*>(d_prof);
   A.reactor = o.reactor;
+  A.fallback = (o.lu_precision == 1 && o.fp64_fallback) ? 1 : 0;
 
   HPO_HIP(hpo::launch_reset_counter(res->d_queue, st));
   if (tm) HPO_HIP(hipEventRecord(m->ev0, st));
@@ -230,6 +257,8 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
       HPO_HIP(hpo::launch_bdf_block(ns, A, o.lu_precision, static_cast<int>(grid), lds, st));
     else
       HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
+    if (A.fallback)
+      launch_fp64_fallback(m, block_layout, A, wpb, block, lds, res->share_pct, res->scratch_doubles, st);
   }
   if (tm) HPO_HIP(hipEventRecord(m->ev1, st));
   if (own_ctx && !capturing) {
@@ -246,6 +275,13 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
     tm->grid = static_cast<int32_t>(grid);
     tm->block = block;
     tm->lds_bytes = static_cast<int32_t>(lds);
+    tm->fallback_cells = 0;
+    if (A.fallback && ncells > 0) {
+      uint32_t q[2] = {0, 0};
+      HPO_HIP(hipMemcpyAsync(q, res->d_queue, sizeof(q), hipMemcpyDeviceToHost, st));
+      HPO_HIP(hipStreamSynchronize(st));
+      tm->fallback_cells = static_cast<int32_t>(q[1]);
+    }
   }
 }
 
@@ -289,6 +325,7 @@ void hpo_opts_default(hpo_opts* o) {
   o->maxsteps = 10000;
   o->lu_precision = 0;
   o->refine_sweeps = 1;
+  o->fp64_fallback = 1;
 }
 
 const char* hpo_last_error(void) { return g_err.c_str(); }
@@ -570,9 +607,15 @@ int hpo_integrate_device_noalloc(const hpo_mech* mc, int64_t ncells, const doubl
     A.refine_sweeps = opt.refine_sweeps;
     A.prof = nullptr;
     A.reactor = opt.reactor;
+    A.fallback = (opt.lu_precision == 1 && opt.fp64_fallback) ? 1 : 0;
     hipStream_t st = static_cast<hipStream_t>(stream);
     HPO_HIP(hpo::launch_reset_counter(A.queue, st));
-    if (ncells > 0) HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
+    if (ncells > 0) {
+      HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
+      if (A.fallback)
+        launch_fp64_fallback(m, false, A, wpb, block, lds, 100,
+                             static_cast<int64_t>((hpo::kNoallocScratchBytes - 256) / sizeof(double)), st);
+    }
     return HPO_OK;
   } catch (...) {
     return map_exceptions();

@@ -41,7 +41,18 @@ struct BdfArgs {
   int64_t* prof;             // [ncells][HPO_NPROF] shader-clock cycles per phase, or null
   BlkLayout blay;            // block-per-cell kernel: layout of the block tables at `mech`
   int32_t reactor;           // 0 constant pressure (P = pressure, Pa); 1 constant volume (P = density, kg/m^3)
+  // FP64 fallback of the FP32 Newton matrix. 1 (the FP32 launch): a cell that ends with a solver
+  // failure (hpo_fallback_status) leaves its output row unwritten and is counted in queue[1].
+  // 2 (the FP64 launch after it): only those cells, restarted from their input rows, are
+  // integrated; chunks of 64 cells are taken from queue[2], and nothing runs when queue[1] is 0.
+  int32_t fallback;
 };
+
+// Solver failures a cell may recover from in FP64: convergence and error-test failures, too
+// much work or too much accuracy asked. Bad input (ILL_INPUT, TOO_CLOSE) is not retried.
+__host__ __device__ inline bool hpo_fallback_status(int s) {
+  return s == HPO_CONV_FAILURE || s == HPO_ERR_FAILURE || s == HPO_TOO_MUCH_WORK || s == HPO_TOO_MUCH_ACC;
+}
 
 struct EvalArgs {
   const uint8_t* mech;

@@ -66,6 +66,10 @@ typedef struct {
                               its pressure in Pa; 1 constant volume (CONV, dydt.c:135-263): each
                               cell's P is its density in kg/m^3, held through the interval, and
                               the temperature equation uses internal energies and cv */
+  int32_t fp64_fallback;   /* with lu_precision == 1: 1 (default) re-integrates in FP64, from its
+                              input state, every cell whose FP32 run ends in a solver failure
+                              (convergence or error-test failure, too much work or accuracy),
+                              in a second launch on the same stream; 0 reports the FP32 failure */
 } hpo_opts;
 
 typedef struct {
@@ -74,7 +78,7 @@ typedef struct {
   int64_t ncells;
   int32_t grid, block;     /* persistent launch shape */
   int32_t lds_bytes;       /* dynamic LDS per workgroup */
-  int32_t reserved;
+  int32_t fallback_cells;  /* cells re-integrated in FP64 after an FP32 failure (fp64_fallback) */
 } hpo_timing;
 
 /* Universal gas constant of the library's thermochemistry, J/(kmol K); e.g. for the density a
