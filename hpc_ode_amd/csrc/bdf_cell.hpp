@@ -29,6 +29,11 @@ enum { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
 enum { PH_F0 = 0, PH_HIN = 1, PH_NEWTON = 2, PH_RESTART = 3 };
 enum { ACT_EVAL = 0, ACT_BEGIN = 1, ACT_ATTEMPT = 2, ACT_DONE = 3 };
 
+// Word 3 of a launch's work queue: nonzero when the FP64 fallback is armed (read alike by every lane).
+HPO_DI bool hpo_fallback_armed(const uint32_t* q) {
+  return __builtin_amdgcn_readfirstlane(static_cast<int>(*reinterpret_cast<const volatile uint32_t*>(q + 3))) != 0;
+}
+
 // CVODE's weighted RMS norm over the cell's components (one per thread of the cell's group).
 template <class Lay>
 HPO_DI double wrms(const Lay& Ly, double v, double w, int n) {
@@ -600,11 +605,8 @@ __device__ __forceinline__ void integrate_cell(const BdfArgs& A, Lay& Ly, Scal& 
       }
     }
   }
-  // with the FP64 fallback armed, a failed FP32 cell keeps its input row for the FP64 launch
-  const bool keep_row = A.fallback == 1 && hpo_fallback_status(status);
-  if (act && !keep_row) A.y_out[cell * A.ld + lane] = yv;
+  if (act) A.y_out[cell * A.ld + lane] = yv;
   if (lane == 0) {
-    if (keep_row) atomicAdd(A.queue + 1, 1u);
     A.status[cell] = status;
     if (A.stats) {
       int* st = A.stats + cell * HPO_NSTATS;

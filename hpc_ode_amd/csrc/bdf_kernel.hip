@@ -199,34 +199,19 @@ __global__ __launch_bounds__(256) __attribute__((
   static_assert((jx_cols<NS>() - 1) * NS + 64 <= 64 * 64, "padded J x columns leave the Jacobian block");
   for (int i = NS * NS + lane; i < (jx_cols<NS>() - 1) * NS + 64; i += 64) g.J[i] = 0.0;
   wave_sync();
-  // FP64 launches after an FP32 one with the fallback armed take only the cells that failed
-  // there, 64 cells per queue pull (their statuses read one per lane), from their input rows
-  const bool fb = !LU32 && A.fallback == 2;
-  if (fb && __builtin_amdgcn_readfirstlane(static_cast<int>(*reinterpret_cast<volatile uint32_t*>(A.queue + 1))) == 0)
+  // An FP64 launch with the fallback armed takes only the cells that the FP32 launch before it
+  // left failed, one per pull from queue[2], from their input rows
+  if (!LU32 && hpo_fallback_armed(A.queue) &&
+      __builtin_amdgcn_readfirstlane(static_cast<int>(*reinterpret_cast<volatile uint32_t*>(A.queue + 1))) == 0)
     return;
-  unsigned long long todo = 0ull;
-  unsigned int c0 = 0;
   for (;;) {
-    int64_t cell;
-    if (fb) {
-      while (todo == 0ull) {
-        unsigned int c = 0;
-        if (lane == 0) c = atomicAdd(A.queue + 2, 64u);
-        c0 = __builtin_amdgcn_readfirstlane(c);
-        if (static_cast<int64_t>(c0) >= A.ncells) break;
-        const int64_t mine = static_cast<int64_t>(c0) + lane;
-        todo = __ballot(mine < A.ncells && hpo_fallback_status(A.status[mine]));
-      }
-      if (todo == 0ull) break;
-      cell = static_cast<int64_t>(c0) + (__ffsll(todo) - 1);
-      todo &= todo - 1;
-    } else {
-      unsigned int c = 0;
-      if (lane == 0) c = atomicAdd(A.queue, 1u);
-      c = __builtin_amdgcn_readfirstlane(c);
-      if (static_cast<int64_t>(c) >= A.ncells) break;
-      cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
-    }
+    const bool fb = !LU32 && hpo_fallback_armed(A.queue);
+    unsigned int c = 0;
+    if (lane == 0) c = atomicAdd(A.queue + (fb ? 2 : 0), 1u);
+    c = __builtin_amdgcn_readfirstlane(c);
+    if (static_cast<int64_t>(c) >= A.ncells) break;
+    const int64_t cell = (!fb && A.order) ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
+    if (fb && !hpo_fallback_status(__builtin_amdgcn_readfirstlane(A.status[cell]))) continue;
     WaveLayout<NS, LU32, INV> Ly(t, w, g, lane);
     integrate_cell<NS, LU32, PROF>(A, Ly, S, cell);
   }
@@ -350,15 +335,36 @@ const void* bdf_kernel_fn(int ns, int variant, int prof, int conv) {
 }
 
 namespace dev {
-// Zeroes a launch's work-queue counters (four words) on the launch's stream. A kernel rather than a memset, so
-// that a caller capturing the launch into a HIP graph records two kernel nodes.
-__global__ __launch_bounds__(64) void reset_counter(uint32_t* q) {
-  if (threadIdx.x < 4) q[threadIdx.x] = 0u;   // the queue, the FP64-fallback count and queue
+// Sets a launch's four work-queue words on the launch's stream: the counters to zero and the
+// FP64-fallback word. A kernel rather than a memset, so that a caller capturing the launch into
+// a HIP graph records only kernel nodes.
+__global__ __launch_bounds__(64) void reset_counter(uint32_t* q, uint32_t armed) {
+  if (threadIdx.x < 4) q[threadIdx.x] = threadIdx.x == 3 ? armed : 0u;
 }
 }  // namespace dev
 
-hipError_t launch_reset_counter(uint32_t* q, hipStream_t st) {
-  hipLaunchKernelGGL(dev::reset_counter, dim3(1), dim3(64), 0, st, q);
+hipError_t launch_reset_counter(uint32_t* q, bool fallback_armed, hipStream_t st) {
+  hipLaunchKernelGGL(dev::reset_counter, dim3(1), dim3(64), 0, st, q, fallback_armed ? 1u : 0u);
+  return hipGetLastError();
+}
+
+namespace dev {
+// Block-strided over the cells; every thread of a block runs the same iterations, so the
+// ballot sees whole waves.
+__global__ __launch_bounds__(256) void count_failed(const int32_t* status, int64_t n, uint32_t* q) {
+  const int64_t stride = static_cast<int64_t>(blockDim.x) * gridDim.x;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const unsigned long long b = __ballot(i < n && hpo_fallback_status(status[i]));
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(q + 1, static_cast<uint32_t>(__popcll(b)));
+  }
+}
+}  // namespace dev
+
+hipError_t launch_count_failed(const int32_t* status, int64_t n, uint32_t* q, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(dev::count_failed, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, status, n, q);
   return hipGetLastError();
 }
 

@@ -812,36 +812,19 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((
   BlkLds w = make_blk_lds(wbase, Lay.K, Lay.R);
   Scal& S = *reinterpret_cast<Scal*>(wbase + blk_lds_doubles(Lay.K, Lay.R) + wave * kScalDoubles);
   BlkScratch g = make_blk_scratch(A.scratch + static_cast<int64_t>(blockIdx.x) * A.scratch_stride, Lay);
-  // FP64 launches after an FP32 one with the fallback armed take only the cells that failed
-  // there, 64 cells per queue pull, from their input rows
-  const bool fb = !LU32 && A.fallback == 2;
-  if (fb && *reinterpret_cast<volatile uint32_t*>(A.queue + 1) == 0) return;   // alike in every thread
-  int64_t c0 = 0;
-  int next = 64;   // position in the current chunk of 64 cells
+  // An FP64 launch with the fallback armed takes only the cells that the FP32 launch before it
+  // left failed, one per pull from queue[2], from their input rows (the words read alike in
+  // every thread)
+  if (!LU32 && hpo_fallback_armed(A.queue) && *reinterpret_cast<volatile uint32_t*>(A.queue + 1) == 0) return;
   for (;;) {
-    int64_t cell = -1;
-    if (fb) {
-      while (cell < 0) {
-        if (next >= 64) {
-          __syncthreads();
-          if (tid == 0) w.iscal[8] = static_cast<int>(atomicAdd(A.queue + 2, 64u));
-          __syncthreads();
-          c0 = static_cast<unsigned>(w.iscal[8]);
-          next = 0;
-          if (c0 >= A.ncells) break;
-        }
-        const int64_t c = c0 + next++;
-        if (c < A.ncells && hpo_fallback_status(A.status[c])) cell = c;   // read alike by every thread
-      }
-      if (cell < 0) break;
-    } else {
-      __syncthreads();
-      if (tid == 0) w.iscal[8] = static_cast<int>(atomicAdd(A.queue, 1u));
-      __syncthreads();
-      const unsigned c = static_cast<unsigned>(w.iscal[8]);
-      if (static_cast<int64_t>(c) >= A.ncells) break;
-      cell = A.order ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
-    }
+    const bool fb = !LU32 && hpo_fallback_armed(A.queue);
+    __syncthreads();
+    if (tid == 0) w.iscal[8] = static_cast<int>(atomicAdd(A.queue + (fb ? 2 : 0), 1u));
+    __syncthreads();
+    const unsigned c = static_cast<unsigned>(w.iscal[8]);
+    if (static_cast<int64_t>(c) >= A.ncells) break;
+    const int64_t cell = (!fb && A.order) ? static_cast<int64_t>(A.order[c]) : static_cast<int64_t>(c);
+    if (fb && !hpo_fallback_status(A.status[cell])) continue;
     __syncthreads();
     blk_plog_cell(t, A.P[cell], tid);   // PLOG rates at this cell's pressure
     __syncthreads();

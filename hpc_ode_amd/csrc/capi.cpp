@@ -51,6 +51,7 @@ hpo_mech::~hpo_mech() {
     if (d_bblob) (void)hipFree(d_bblob);
     if (d_scratch) (void)hipFree(d_scratch);
     if (d_queue) (void)hipFree(d_queue);
+    if (d_backup) (void)hipFree(d_backup);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev_last) (void)hipEventDestroy(ev_last);
@@ -89,6 +90,15 @@ void hpo_mech::ensure_scratch_doubles(int64_t n) {
 }
 
 void hpo_mech::ensure_scratch(int64_t slots) { ensure_scratch_doubles(slots * scratch_stride); }
+
+void hpo_mech::ensure_backup_doubles(int64_t n) {
+  if (n <= backup_cap) return;
+  if (launched) HPO_HIP(hipEventSynchronize(ev_last));
+  if (d_backup) HPO_HIP(hipFree(d_backup));
+  d_backup = nullptr;
+  d_backup = dalloc<double>(static_cast<size_t>(n));
+  backup_cap = n;
+}
 
 struct hpo_batch {
   const hpo_mech* mech = nullptr;
@@ -141,13 +151,23 @@ bool use_block_layout(const hpo_mech* m, const hpo_opts& o) {
   throw std::invalid_argument("no kernel instantiated for a state of " + std::to_string(ns) + " variables");
 }
 
-// The FP64 launch that follows an FP32 one whose cells had the fallback armed (A.fallback was 1):
-// the same arguments in fallback mode 2, on a grid that fits the FP64 kernel's occupancy, the
-// share of the device and the scratch the FP32 launch used (the FP64 kernel never holds more
-// waves per CU than the FP32 one, so the grid is rarely cut by it).
-static void launch_fp64_fallback(const hpo_mech* m, bool block_layout, hpo::BdfArgs A, int wpb, int block,
-                                 size_t lds, int share_pct, int64_t scratch_doubles, hipStream_t st) {
+// Whether two arrays of n doubles share memory.
+static bool rows_overlap(const double* a, const double* b, int64_t n) {
+  const uintptr_t ua = reinterpret_cast<uintptr_t>(a), ub = reinterpret_cast<uintptr_t>(b);
+  const uintptr_t bytes = static_cast<uintptr_t>(n) * sizeof(double);
+  return ua < ub + bytes && ub < ua + bytes;
+}
+
+// The FP64 launch that follows an FP32 one with the fallback armed (queue word 3 set by
+// launch_reset_counter): count_failed, then the same arguments with the input rows at
+// d_rows_in (the FP32 launch's own input, or its copy when it ran in place), on a grid that fits
+// the FP64 kernel's occupancy, the share of the device and the scratch the FP32 launch used (the
+// FP64 kernel never holds more waves per CU than the FP32 one, so the grid is rarely cut by it).
+static void launch_fp64_fallback(const hpo_mech* m, bool block_layout, hpo::BdfArgs A, const double* d_rows_in,
+                                 int wpb, int block, size_t lds, int share_pct, int64_t scratch_doubles,
+                                 hipStream_t st) {
   const int ns = m->host.K();
+  HPO_HIP(hpo::launch_count_failed(A.status, A.ncells, A.queue, st));
   const bool prof = A.prof != nullptr;
   const void* fn = block_layout ? hpo::bdf_block_kernel_fn(ns, 0, prof) : hpo::bdf_kernel_fn(ns, 0, prof, A.reactor);
   int occ = 0;
@@ -158,8 +178,8 @@ static void launch_fp64_fallback(const hpo_mech* m, bool block_layout, hpo::BdfA
   grid = std::min<int64_t>(grid, (A.ncells + wpb - 1) / wpb);
   grid = std::min<int64_t>(grid, scratch_doubles / (static_cast<int64_t>(wpb) * A.scratch_stride));
   grid = std::max<int64_t>(grid, 1);
-  A.fallback = 2;
   A.order = nullptr;
+  A.y_in = d_rows_in;
   if (block_layout)
     HPO_HIP(hpo::launch_bdf_block(ns, A, 0, static_cast<int>(grid), lds, st));
   else
@@ -248,17 +268,35 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
   A.refine_sweeps = o.refine_sweeps;
   A.prof = reinterpret_cast<int64_t*>(d_prof);
   A.reactor = o.reactor;
-  A.fallback = (o.lu_precision == 1 && o.fp64_fallback) ? 1 : 0;
+  const bool fallback = o.lu_precision == 1 && o.fp64_fallback && ncells > 0;
+  // The FP64 fallback restarts failed cells from their input rows: an in-place launch keeps a
+  // copy of them (the FP32 kernel writes every output row).
+  const double* rows_in = d_yin;
+  const int64_t rows = ncells * ld;
+  if (fallback && rows_overlap(d_yin, d_yout, rows)) {
+    double* backup = nullptr;
+    if (own_ctx) {
+      m->ensure_backup_doubles(rows);
+      backup = m->d_backup;
+    } else {
+      if (!res->d_backup || res->backup_doubles < rows)
+        throw std::invalid_argument("launch context has no room for the input rows of the FP64 fallback");
+      backup = res->d_backup;
+    }
+    rows_in = backup;
+  }
 
-  HPO_HIP(hpo::launch_reset_counter(res->d_queue, st));
+  HPO_HIP(hpo::launch_reset_counter(res->d_queue, fallback, st));
   if (tm) HPO_HIP(hipEventRecord(m->ev0, st));
+  if (rows_in != d_yin) HPO_HIP(hipMemcpyAsync(const_cast<double*>(rows_in), d_yin, rows * sizeof(double),
+                                                hipMemcpyDeviceToDevice, st));
   if (ncells > 0) {
     if (block_layout)
       HPO_HIP(hpo::launch_bdf_block(ns, A, o.lu_precision, static_cast<int>(grid), lds, st));
     else
       HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
-    if (A.fallback)
-      launch_fp64_fallback(m, block_layout, A, wpb, block, lds, res->share_pct, res->scratch_doubles, st);
+    if (fallback)
+      launch_fp64_fallback(m, block_layout, A, rows_in, wpb, block, lds, res->share_pct, res->scratch_doubles, st);
   }
   if (tm) HPO_HIP(hipEventRecord(m->ev1, st));
   if (own_ctx && !capturing) {
@@ -276,7 +314,7 @@ void launch_integrate(hpo_mech* m, int64_t ncells, const double* d_yin, double* 
     tm->block = block;
     tm->lds_bytes = static_cast<int32_t>(lds);
     tm->fallback_cells = 0;
-    if (A.fallback && ncells > 0) {
+    if (fallback) {
       uint32_t q[2] = {0, 0};
       HPO_HIP(hipMemcpyAsync(q, res->d_queue, sizeof(q), hipMemcpyDeviceToHost, st));
       HPO_HIP(hipStreamSynchronize(st));
@@ -607,13 +645,17 @@ int hpo_integrate_device_noalloc(const hpo_mech* mc, int64_t ncells, const doubl
     A.refine_sweeps = opt.refine_sweeps;
     A.prof = nullptr;
     A.reactor = opt.reactor;
-    A.fallback = (opt.lu_precision == 1 && opt.fp64_fallback) ? 1 : 0;
+    // the FP64 fallback needs the input rows after the FP32 launch: not in place here, where
+    // there is no memory to copy them to
+    const int64_t rows = ncells * ld;
+    const bool in_place = rows_overlap(d_y_in, d_y_out, rows);
+    const bool fallback = opt.lu_precision == 1 && opt.fp64_fallback && ncells > 0 && !in_place;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    HPO_HIP(hpo::launch_reset_counter(A.queue, st));
+    HPO_HIP(hpo::launch_reset_counter(A.queue, fallback, st));
     if (ncells > 0) {
       HPO_HIP(hpo::launch_bdf(ns, A, variant, static_cast<int>(grid), block, lds, st));
-      if (A.fallback)
-        launch_fp64_fallback(m, false, A, wpb, block, lds, 100,
+      if (fallback)
+        launch_fp64_fallback(m, false, A, d_y_in, wpb, block, lds, 100,
                              static_cast<int64_t>((hpo::kNoallocScratchBytes - 256) / sizeof(double)), st);
     }
     return HPO_OK;

@@ -46,6 +46,9 @@ struct ExecRes {
   double* d_scratch = nullptr;
   int64_t scratch_doubles = 0;  // capacity
   int32_t share_pct = 100;      // largest share of n_CU x workgroups-per-CU the grid may take
+  // copy of the input rows of an in-place launch, for its FP64 fallback (null: none)
+  double* d_backup = nullptr;
+  int64_t backup_doubles = 0;   // capacity
 };
 
 // Which kernel layout a launch of this mechanism with these options uses: true = one workgroup
@@ -82,6 +85,8 @@ struct hpo_mech {
   int64_t scratch_cap = 0;      // doubles
   int64_t scratch_stride = 0;   // doubles per wave slot of the wave layout
   uint32_t* d_queue = nullptr;
+  double* d_backup = nullptr;   // input rows of in-place launches, for the FP64 fallback
+  int64_t backup_cap = 0;       // doubles
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // Launches on the handle's own queue counter and scratch are ordered on the device: each
   // waits for the previous one (ev_last, recorded on last_stream) when it is issued on another
@@ -95,5 +100,6 @@ struct hpo_mech {
   void ensure_device();               // uploads the tables on first use (current device)
   void ensure_scratch(int64_t slots);  // wave slots of the wave layout
   void ensure_scratch_doubles(int64_t n);
+  void ensure_backup_doubles(int64_t n);
   int state_size() const { return host.K(); }
 };

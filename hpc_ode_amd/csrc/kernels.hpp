@@ -41,11 +41,6 @@ struct BdfArgs {
   int64_t* prof;             // [ncells][HPO_NPROF] shader-clock cycles per phase, or null
   BlkLayout blay;            // block-per-cell kernel: layout of the block tables at `mech`
   int32_t reactor;           // 0 constant pressure (P = pressure, Pa); 1 constant volume (P = density, kg/m^3)
-  // FP64 fallback of the FP32 Newton matrix. 1 (the FP32 launch): a cell that ends with a solver
-  // failure (hpo_fallback_status) leaves its output row unwritten and is counted in queue[1].
-  // 2 (the FP64 launch after it): only those cells, restarted from their input rows, are
-  // integrated; chunks of 64 cells are taken from queue[2], and nothing runs when queue[1] is 0.
-  int32_t fallback;
 };
 
 // Solver failures a cell may recover from in FP64: convergence and error-test failures, too
@@ -53,6 +48,14 @@ struct BdfArgs {
 __host__ __device__ inline bool hpo_fallback_status(int s) {
   return s == HPO_CONV_FAILURE || s == HPO_ERR_FAILURE || s == HPO_TOO_MUCH_WORK || s == HPO_TOO_MUCH_ACC;
 }
+
+// The work queue of a launch, four words: [0] the cell counter of a plain launch, [1] the cells
+// an FP32 launch left failed (hpo_fallback_status), counted by count_failed after it, [2] the
+// cell counter of the FP64 launch that retries them, [3] nonzero when that FP64 fallback is
+// armed. Armed, an FP64 launch integrates only the failed cells, from the input rows it is
+// given, one per pull from [2]; it does nothing when [1] is 0. The kernels read word 3 from
+// memory where they need it rather than as an argument, so that no value of it stays live
+// across a cell's integration.
 
 struct EvalArgs {
   const uint8_t* mech;
@@ -83,7 +86,9 @@ int64_t wave_scratch_doubles_host(const DevMechLayout& L);
 // variant: 0 FP64 LU; 1 FP32 explicit inverse (Gauss-Jordan); 2 FP32 in-place LU
 const void* bdf_kernel_fn(int ns, int variant, int prof = 0, int conv = 0);
 hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block, size_t shmem, hipStream_t st);
-hipError_t launch_reset_counter(uint32_t* q, hipStream_t st);
+hipError_t launch_reset_counter(uint32_t* q, bool fallback_armed, hipStream_t st);
+// q[1] += the cells of status[0..n) that ended in a solver failure an FP64 run may recover from
+hipError_t launch_count_failed(const int32_t* status, int64_t n, uint32_t* q, hipStream_t st);
 // The no-allocation entry point's fixed scratch: 192 MiB of static device memory per GPU (the
 // counter in the first 256 bytes, then the wave slots), enough for a whole-device launch of a
 // 53-variable FP32 kernel at two waves per SIMD. noalloc_scratch returns its address on the

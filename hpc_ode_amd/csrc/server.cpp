@@ -8,8 +8,9 @@
 // in use are warm (a hipFree waits for the whole device, which here is kept busy by the
 // launches). Every launch slot owns a HIP stream, a work-queue counter, per-wave scratch
 // sized for a whole-device launch of any of the server's mechanisms, and packing buffers of
-// max_batch_cells rows (device) plus pinned host staging. Stream uploads and downloads run on
-// a dedicated non-blocking stream, never on the null stream.
+// max_batch_cells rows (device; the rows twice with the FP64 fallback, which restarts failed
+// cells of an in-place launch from a copy of its input) plus pinned host staging. Stream
+// uploads and downloads run on a dedicated non-blocking stream, never on the null stream.
 //
 // A launch packs its requests' rows into the slot buffers (device-to-device), integrates them
 // in place with the persistent BDF kernel, scatters them back to their streams and stages the
@@ -147,7 +148,7 @@ struct hpo_server {
     for (auto& sl : slots) {
       for (void* p : {static_cast<void*>(sl.d_y), static_cast<void*>(sl.d_P), static_cast<void*>(sl.d_dt),
                       static_cast<void*>(sl.d_status), static_cast<void*>(sl.res.d_queue),
-                      static_cast<void*>(sl.res.d_scratch)})
+                      static_cast<void*>(sl.res.d_scratch), static_cast<void*>(sl.res.d_backup)})
         if (p) (void)hipFree(p);
       for (void* p : {static_cast<void*>(sl.h_y), static_cast<void*>(sl.h_dt), static_cast<void*>(sl.h_status)})
         if (p) (void)hipHostFree(p);
@@ -359,6 +360,11 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
       sl.res.share_pct = s->opts.cu_share_pct;
       sl.res.d_scratch = dalloc<double>(static_cast<size_t>(scratch));
       sl.d_y = dalloc<double>(static_cast<size_t>(sl.cap * ldmax));
+      if (s->opts.opts.lu_precision == 1 && s->opts.opts.fp64_fallback) {
+        // launches run in place: the FP64 fallback restarts failed cells from a copy of the rows
+        sl.res.backup_doubles = sl.cap * ldmax;
+        sl.res.d_backup = dalloc<double>(static_cast<size_t>(sl.res.backup_doubles));
+      }
       sl.d_P = dalloc<double>(sl.cap);
       sl.d_dt = dalloc<double>(sl.cap);
       sl.d_status = dalloc<int32_t>(sl.cap);

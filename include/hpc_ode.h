@@ -69,7 +69,9 @@ typedef struct {
   int32_t fp64_fallback;   /* with lu_precision == 1: 1 (default) re-integrates in FP64, from its
                               input state, every cell whose FP32 run ends in a solver failure
                               (convergence or error-test failure, too much work or accuracy),
-                              in a second launch on the same stream; 0 reports the FP32 failure */
+                              in a second launch on the same stream; 0 reports the FP32 failure.
+                              An in-place call (y_in and y_out overlap) first copies the input
+                              rows to device memory the library keeps for it */
 } hpo_opts;
 
 typedef struct {
@@ -150,7 +152,8 @@ int hpo_integrate_device(const hpo_mech *m, int64_t ncells, const double *d_y_in
  * on the current device (any allocating integration call uploads them; hpo_integrate_device with
  * ncells = 0 does nothing else). Every call on one GPU shares that GPU's array and clears its
  * counter with a kernel on hip_stream: calls that overlap on one GPU race, so order them on one
- * stream. It returns after the launch (no timing, no profile). */
+ * stream. It returns after the launch (no timing, no profile). In place (d_y_in and d_y_out
+ * overlap) it runs without the FP64 fallback, which would need a copy of the input rows. */
 int hpo_integrate_device_noalloc(const hpo_mech *m, int64_t ncells, const double *d_y_in, double *d_y_out,
                                  int64_t ld, const double *d_P, const double *d_dt, int32_t *d_status,
                                  int32_t *d_stats, const int32_t *d_order, const hpo_opts *o,

@@ -1,13 +1,17 @@
 """The FP64 fallback of the FP32 Newton matrix (hpo_opts::fp64_fallback, on by default): a cell
 whose FP32 run ends in a solver failure (convergence or error-test failure, too much work, too
-much accuracy) keeps its input row, and a second launch on the same stream re-integrates it in
-FP64 from there. So with the fallback each cell's result is either its FP32 result (when that
-succeeded) or, bit for bit, the plain FP64 kernel's result for the cell.
+much accuracy) is re-integrated in FP64 from its input row by a second launch on the same
+stream (an in-place call first copies its input rows). So with the fallback each cell's result
+is either its FP32 result (when that succeeded) or, bit for bit, the plain FP64 kernel's result
+for the cell, whichever entry point ran it: the batch (in place), device arrays (separate
+input and output), the serving loop (in place in its slots). The no-allocation entry point runs
+without the fallback in place and with it otherwise.
 
 A step limit (maxsteps) below some cells' step counts makes those cells fail with
 TOO_MUCH_WORK in FP32 and again in FP64 (where the FP64 result is the state after the same
 number of FP64 steps), which exercises the mechanism on chosen cells without forcing a
 numerical failure."""
+import ctypes as C
 import os
 
 import numpy as np
@@ -65,3 +69,91 @@ def test_no_failure_no_change():
     yb, sb, stb, tb = _run(m, y0, P, dt, lu_precision=1, fp64_fallback=0)
     assert (sa == 0).all() and ta.fallback_cells == 0
     assert np.array_equal(ya, yb) and np.array_equal(sta["nst"], stb["nst"])
+
+
+def _hip():
+    H.lib()   # loads libamdhip64.so.7 as a dependency of libhpc_ode.so
+    hip = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+    vp = C.c_void_p
+    hip.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+    hip.hipDeviceSynchronize.argtypes = []
+    return hip
+
+
+def test_fallback_through_device_arrays_and_serving():
+    """The same limited-step GRI cells through hpo_integrate_device (separate input and output
+    arrays, then in place), hpo_integrate_device_noalloc (separate: with the fallback; in place:
+    without it) and the serving loop, against the batch's results."""
+    m = H.Mechanism(os.path.join(MECH, "gri30.yaml"))
+    n, ns = 384, m.n_state
+    y0, P, dt = WL.gri_plain(m.species, m.W, n, seed=21)
+    y0 = np.ascontiguousarray(y0)
+    _, _, st32, _ = _run(m, y0, P, dt, lu_precision=1, fp64_fallback=0)
+    limit = int(np.median(st32["nst"]))
+    yf, sf, _, tf = _run(m, y0, P, dt, lu_precision=1, maxsteps=limit)
+    yn, sn, _, _ = _run(m, y0, P, dt, lu_precision=1, maxsteps=limit, fp64_fallback=0)
+    failed = sn == TOO_MUCH_WORK
+    assert 0 < failed.sum() < n and tf.fallback_cells == failed.sum()
+    assert not np.array_equal(yf[failed], yn[failed])
+
+    hip, L = _hip(), H.lib()
+    bufs = []
+
+    def dev(nbytes):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), nbytes) == 0
+        bufs.append(p)
+        return p
+
+    def put(p, a):
+        assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+
+    def get(p, shape, dtype):
+        a = np.empty(shape, dtype)
+        assert hip.hipMemcpy(a.ctypes.data, p, a.nbytes, 2) == 0
+        return a
+
+    o = H.opts(lu_precision=1, maxsteps=limit)
+    try:
+        d_y0, d_y, d_P, d_dt, d_s = dev(y0.nbytes), dev(y0.nbytes), dev(P.nbytes), dev(dt.nbytes), dev(4 * n)
+        put(d_y0, y0), put(d_P, P), put(d_dt, dt)
+        print("hpo_integrate_device, separate arrays", flush=True)
+        H._check(L.hpo_integrate_device(m.handle, n, d_y0, d_y, ns, d_P, d_dt, d_s, None, None, None,
+                                        C.byref(o), None, None))
+        assert hip.hipDeviceSynchronize() == 0
+        assert np.array_equal(get(d_y, (n, ns), np.float64), yf) and np.array_equal(get(d_s, n, np.int32), sf)
+        print("hpo_integrate_device, in place", flush=True)
+        put(d_y, y0)
+        H._check(L.hpo_integrate_device(m.handle, n, d_y, d_y, ns, d_P, d_dt, d_s, None, None, None,
+                                        C.byref(o), None, None))
+        assert hip.hipDeviceSynchronize() == 0
+        assert np.array_equal(get(d_y, (n, ns), np.float64), yf) and np.array_equal(get(d_s, n, np.int32), sf)
+        print("hpo_integrate_device_noalloc, separate arrays", flush=True)
+        H._check(L.hpo_integrate_device_noalloc(m.handle, n, d_y0, d_y, ns, d_P, d_dt, d_s, None, None,
+                                                C.byref(o), None))
+        assert hip.hipDeviceSynchronize() == 0
+        assert np.array_equal(get(d_y, (n, ns), np.float64), yf) and np.array_equal(get(d_s, n, np.int32), sf)
+        print("hpo_integrate_device_noalloc, in place", flush=True)
+        put(d_y, y0)
+        H._check(L.hpo_integrate_device_noalloc(m.handle, n, d_y, d_y, ns, d_P, d_dt, d_s, None, None,
+                                                C.byref(o), None))
+        assert hip.hipDeviceSynchronize() == 0
+        assert np.array_equal(get(d_y, (n, ns), np.float64), yn) and np.array_equal(get(d_s, n, np.int32), sn)
+    finally:
+        hip.hipDeviceSynchronize()
+        for p in bufs:
+            hip.hipFree(p)
+
+    print("serving loop", flush=True)
+    srv = H.Server([m], n_slots=2, lu_precision=1, maxsteps=limit)
+    try:
+        sid = srv.open_stream(0, y0, P)
+        ys, ss, info = srv.wait(srv.submit(sid, float(dt[0]), n=n, n_state=ns))
+        srv.close_stream(sid)
+    finally:
+        srv.close()
+    assert (dt == dt[0]).all()
+    assert info["failed_cells"] == (sf != 0).sum()
+    assert np.array_equal(ys, yf) and np.array_equal(ss, sf)
