@@ -59,7 +59,7 @@ def parse():
                     help="offered loads as fractions of the measured serving capacity")
     ap.add_argument("--serve-substeps", type=int, default=1000,
                     help="serving: each load runs until it has offered at least this many substeps")
-    ap.add_argument("--serve-slots", type=int, default=4, help="serving: launch slots (1..16)")
+    ap.add_argument("--serve-slots", type=int, default=8, help="serving: launch slots (1..16)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -80,7 +80,7 @@ def measured_traffic(workload, ncells, lu_precision):
     return rec.get("traffic_bytes")
 
 
-def serving_measurement(H, gri, seconds, fractions, cu_share=0, min_substeps=1000, slots=4):
+def serving_measurement(H, gri, seconds, fractions, cu_share=0, min_substeps=1000, slots=8):
     """p50/p99 substep latency of the serving loop at offered loads stated as fractions of its
     measured capacity (requests: 70 % H2/air, 30 % GRI-3.0, log-uniform [1, 65536] cells,
     dt U[1e-5, 1e-4] s, two substeps per request stream, Poisson arrivals), each load offered

@@ -327,11 +327,12 @@ class Batch:
 class Server:
     """Serving loop (hpo_server): resident cell streams, substep requests, continuous batching."""
 
-    def __init__(self, mechs, n_slots=4, max_batch_cells=262144, hbm_budget_bytes=0, cu_share_pct=0, **opt_kw):
+    def __init__(self, mechs, n_slots=8, max_batch_cells=262144, hbm_budget_bytes=0, cu_share_pct=0, **opt_kw):
         so = ServerOpts()
         lib().hpo_server_opts_default(C.byref(so))
         so.n_slots, so.max_batch_cells, so.hbm_budget_bytes = n_slots, max_batch_cells, hbm_budget_bytes
         so.cu_share_pct = cu_share_pct
+        opt_kw.setdefault("sort_cells", 1)   # the server's default: hottest cells first in each launch
         so.opts = opts(**opt_kw)
         self.mechs = list(mechs)
         arr = (C.c_void_p * len(self.mechs))(*[m.handle.value for m in self.mechs])

@@ -45,6 +45,7 @@ SOURCES = [
     ("bdf_kernel.hip", HIP_FLAGS),
     ("block_kernel.hip", HIP_FLAGS),
     ("lu_block.hip", HIP_FLAGS),
+    ("queue_order.hip", HIP_FLAGS),
 ]
 
 

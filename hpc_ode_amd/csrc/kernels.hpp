@@ -89,6 +89,12 @@ hipError_t launch_bdf(int ns, const BdfArgs& A, int variant, int grid, int block
 hipError_t launch_reset_counter(uint32_t* q, bool fallback_armed, hipStream_t st);
 // q[1] += the cells of status[0..n) that ended in a solver failure an FP64 run may recover from
 hipError_t launch_count_failed(const int32_t* status, int64_t n, uint32_t* q, hipStream_t st);
+// Device-built work-queue order (queue_order.hip): order[0..n) = the cells sorted by descending
+// initial temperature y[i*ld], by buckets of kOrderBuckets over [kOrderTmin, kOrderTmax) K.
+// work: kOrderBuckets words of device memory.
+constexpr int kOrderBuckets = 1024;
+constexpr double kOrderTmin = 200.0, kOrderTmax = 4000.0;
+hipError_t launch_t0_order(const double* y, int64_t n, int64_t ld, uint32_t* work, int32_t* order, hipStream_t st);
 // The no-allocation entry point's fixed scratch: 192 MiB of static device memory per GPU (the
 // counter in the first 256 bytes, then the wave slots), enough for a whole-device launch of a
 // 53-variable FP32 kernel at two waves per SIMD. noalloc_scratch returns its address on the

@@ -12,8 +12,9 @@
 // cells of an in-place launch from a copy of its input) plus pinned host staging. Stream
 // uploads and downloads run on a dedicated non-blocking stream, never on the null stream.
 //
-// A launch packs its requests' rows into the slot buffers (device-to-device), integrates them
-// in place with the persistent BDF kernel, scatters them back to their streams and stages the
+// A launch packs its requests' rows into the slot buffers (device-to-device), orders them by
+// descending initial temperature on the device (opts.sort_cells = 1, the default), integrates
+// them in place with the persistent BDF kernel, scatters them back to their streams and stages the
 // rows callers asked for in pinned memory; an event marks the end. The scheduler thread polls
 // the slots' events, completes finished batches (copies staged rows to caller buffers, wakes
 // waiters) and fills free slots from the pending queue in arrival order. It holds the server
@@ -117,6 +118,8 @@ struct Slot {
   int64_t cap = 0, ldmax = 0;
   double *d_y = nullptr, *d_P = nullptr, *d_dt = nullptr;
   int32_t* d_status = nullptr;
+  int32_t* d_order = nullptr;     // work-queue order of the packed cells (sort_cells = 1)
+  uint32_t* d_owork = nullptr;    // its bucket counters
   double *h_y = nullptr, *h_dt = nullptr;   // pinned
   int32_t* h_status = nullptr;              // pinned
   std::vector<SrvReq*> batch;
@@ -148,7 +151,8 @@ struct hpo_server {
     for (auto& sl : slots) {
       for (void* p : {static_cast<void*>(sl.d_y), static_cast<void*>(sl.d_P), static_cast<void*>(sl.d_dt),
                       static_cast<void*>(sl.d_status), static_cast<void*>(sl.res.d_queue),
-                      static_cast<void*>(sl.res.d_scratch), static_cast<void*>(sl.res.d_backup)})
+                      static_cast<void*>(sl.res.d_scratch), static_cast<void*>(sl.res.d_backup),
+                      static_cast<void*>(sl.d_order), static_cast<void*>(sl.d_owork)})
         if (p) (void)hipFree(p);
       for (void* p : {static_cast<void*>(sl.h_y), static_cast<void*>(sl.h_dt), static_cast<void*>(sl.h_status)})
         if (p) (void)hipHostFree(p);
@@ -188,7 +192,12 @@ struct hpo_server {
       off += st.n;
     }
     HPO_HIP(hipMemcpyAsync(sl.d_dt, sl.h_dt, total * sizeof(double), hipMemcpyHostToDevice, sl.stream));
-    launch_integrate(m, total, sl.d_y, sl.d_y, ld, sl.d_P, sl.d_dt, sl.d_status, nullptr, nullptr, nullptr,
+    const int32_t* order = nullptr;
+    if (opts.opts.sort_cells == 1 && total > 1) {
+      HPO_HIP(hpo::launch_t0_order(sl.d_y, total, ld, sl.d_owork, sl.d_order, sl.stream));
+      order = sl.d_order;
+    }
+    launch_integrate(m, total, sl.d_y, sl.d_y, ld, sl.d_P, sl.d_dt, sl.d_status, nullptr, order, nullptr,
                      opts.opts, sl.stream, nullptr, &sl.res);
     for (SrvReq* r : batch) {
       SrvStream& st = *r->st;
@@ -315,9 +324,10 @@ extern "C" {
 void hpo_server_opts_default(hpo_server_opts* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
-  o->n_slots = 4;
+  o->n_slots = 8;
   o->max_batch_cells = 262144;
   hpo_opts_default(&o->opts);
+  o->opts.sort_cells = 1;   // each launch pulls its hottest cells first (an order built on the device)
 }
 
 int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_opts* o, hpo_server** out) {
@@ -327,7 +337,7 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
   try {
     hpo_server_opts_default(&s->opts);
     if (o) s->opts = *o;
-    if (s->opts.n_slots <= 0) s->opts.n_slots = 4;
+    if (s->opts.n_slots <= 0) s->opts.n_slots = 8;
     if (s->opts.n_slots > 16) throw std::invalid_argument("n_slots must be at most 16");
     if (s->opts.max_batch_cells <= 0) s->opts.max_batch_cells = 262144;
     if (s->opts.cu_share_pct == 0) s->opts.cu_share_pct = 100;
@@ -368,6 +378,8 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
       sl.d_P = dalloc<double>(sl.cap);
       sl.d_dt = dalloc<double>(sl.cap);
       sl.d_status = dalloc<int32_t>(sl.cap);
+      sl.d_order = dalloc<int32_t>(sl.cap);
+      sl.d_owork = dalloc<uint32_t>(hpo::kOrderBuckets);
       HPO_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_y), sl.cap * ldmax * sizeof(double), hipHostMallocDefault));
       HPO_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_dt), sl.cap * sizeof(double), hipHostMallocDefault));
       HPO_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_status), sl.cap * sizeof(int32_t), hipHostMallocDefault));

@@ -203,13 +203,15 @@ int hpo_batch_autotune(hpo_batch *b, int64_t count, int64_t sample_cells, hpo_op
 typedef struct hpo_server hpo_server;
 typedef struct {
   int32_t n_slots;           /* launches that may be in flight at once (one HIP stream each);
-                                1..16, 0 = 4 */
+                                1..16, 0 = 8 */
   int32_t cu_share_pct;      /* largest share of the device's workgroup slots one launch may
                                 take, in percent (1..100); 0 = 100. Below 100, a launch that
                                 arrives while a large one runs finds free slots at once. */
   int64_t max_batch_cells;   /* cells per launch; 0 = 262144 */
   int64_t hbm_budget_bytes;  /* device memory for resident streams; 0 = 90% of free HBM at creation */
-  hpo_opts opts;             /* integration options for every launch */
+  hpo_opts opts;             /* integration options for every launch; sort_cells = 1 (the
+                                default here) orders each launch's cells by descending initial
+                                temperature on the device, 0 takes them in packing order */
 } hpo_server_opts;
 typedef struct {
   double latency_ms;         /* submit -> results available to the caller */

@@ -181,3 +181,23 @@ def test_sixteen_slots_under_load(mechs):
         assert r["failed_cells"] == 0
     finally:
         srv.close()
+
+
+def test_device_order_changes_no_result(mechs):
+    """Each launch's work-queue order is built on the device (hottest cells first, the default):
+    every cell is integrated once, to the same result as in packing order."""
+    gri = mechs[1]
+    y0, P = _state(gri, 1, 5000, 41)
+    out = []
+    for sort in (1, 0):
+        srv = H.Server(mechs, n_slots=2, sort_cells=sort)
+        try:
+            sid = srv.open_stream(1, y0, P)
+            y, s, _ = srv.wait(srv.submit(sid, 6e-5, n=5000, n_state=gri.n_state))
+            srv.close_stream(sid)
+        finally:
+            srv.close()
+        assert (s == 0).all()
+        out.append(y)
+    assert np.array_equal(out[0], out[1])
+    assert not np.array_equal(out[0][:, 0], y0[:, 0])
