@@ -6,7 +6,10 @@ mechanism (70 %) or GRI-3.0 (30 %), with states from the H2/air and CH4/air gene
 hpc_ode_amd.workloads. The stream is opened on the server (admitted against its HBM budget;
 a refused open is retried), advances `substeps` chemistry substeps of dt ~ U[1e-5, 1e-4] s
 with its state resident on the device between them, returns its final state and is closed.
-The latency of a substep is the server's submit-to-result time.
+The latency of a substep is the server's submit-to-result time. The requests' initial states are
+drawn before the load starts, and the summary states how late the client threads started
+requests against their arrival times (client_start_delay_ms_p99), so that a load generator that
+falls behind shows.
 """
 from __future__ import annotations
 
@@ -58,15 +61,21 @@ def run_load(server, mechs, rate_hz, duration_s, seed=0, substeps=2, workers=192
     times = times[:n]
     duration_s = max(duration_s, float(times[-1])) if n else duration_s
     cells, mech, dt = request_mix(n, seed + 1, max_cells)
-    lat, queue, bsz, breq = [], [], [], []
+    # every request's initial state is drawn before the clock starts, so the client threads do
+    # nothing at arrival but call the server (drawing them on arrival held the interpreter long
+    # enough to start 1 % of the requests more than a second late)
+    states = [initial_states(mechs[mech[i]], mech[i], int(cells[i]), seed * 1000003 + i) for i in range(n)]
+    lat, queue, bsz, breq, late = [], [], [], [], []
     failed = [0]
     integrated = [0]
     refused = [0]
     lock = threading.Lock()
 
     def one(i):
+        with lock:
+            late.append(time.perf_counter() - t0 - times[i])
         m = mechs[mech[i]]
-        y0, P = initial_states(m, mech[i], int(cells[i]), seed * 1000003 + i)
+        y0, P = states[i]
         sid = None
         while sid is None:
             sid = server.open_stream(int(mech[i]), y0, P)
@@ -111,6 +120,7 @@ def run_load(server, mechs, rate_hz, duration_s, seed=0, substeps=2, workers=192
         "latency_ms_p99": float(np.percentile(lat, 99)) if lat.size else None,
         "latency_ms_max": float(lat.max()) if lat.size else None,
         "queue_ms_p50": float(np.percentile(queue, 50)) if queue else None,
+        "client_start_delay_ms_p99": float(np.percentile(late, 99) * 1e3) if late else None,
         "mean_batch_cells": float(np.mean(bsz)) if bsz else None,
         "mean_batch_requests": float(np.mean(breq)) if breq else None,
         "cells_per_s": integrated[0] / wall,
