@@ -115,7 +115,7 @@ def test_server_option_checks_come_before_device_calls(h2_path):
     for field, bad in (("cu_share_pct", 101), ("cu_share_pct", -5), ("n_slots", 17)):
         so = H.ServerOpts()
         L.hpo_server_opts_default(C.byref(so))
-        assert so.cu_share_pct == 0 and so.n_slots == 4
+        assert so.cu_share_pct == 0 and so.n_slots == 8 and so.opts.sort_cells == 1
         setattr(so, field, bad)
         h = C.c_void_p()
         assert L.hpo_server_create(arr, 1, C.byref(so), C.byref(h)) == -1
