@@ -66,7 +66,7 @@ class TuneReport(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("n_slots", C.c_int32), ("cu_share_pct", C.c_int32), ("max_batch_cells", C.c_int64),
-                ("hbm_budget_bytes", C.c_int64), ("opts", Opts)]
+                ("hbm_budget_bytes", C.c_int64), ("opts", Opts), ("launch_cells", C.c_int64)]
 
 
 class RequestInfo(C.Structure):
@@ -327,11 +327,14 @@ class Batch:
 class Server:
     """Serving loop (hpo_server): resident cell streams, substep requests, continuous batching."""
 
-    def __init__(self, mechs, n_slots=8, max_batch_cells=262144, hbm_budget_bytes=0, cu_share_pct=0, **opt_kw):
+    def __init__(self, mechs, n_slots=8, max_batch_cells=262144, hbm_budget_bytes=0, cu_share_pct=0, launch_cells=None,
+                 **opt_kw):
         so = ServerOpts()
         lib().hpo_server_opts_default(C.byref(so))
         so.n_slots, so.max_batch_cells, so.hbm_budget_bytes = n_slots, max_batch_cells, hbm_budget_bytes
         so.cu_share_pct = cu_share_pct
+        if launch_cells is not None:
+            so.launch_cells = launch_cells
         opt_kw.setdefault("sort_cells", 1)   # the server's default: hottest cells first in each launch
         so.opts = opts(**opt_kw)
         self.mechs = list(mechs)

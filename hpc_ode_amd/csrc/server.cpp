@@ -30,6 +30,7 @@
 //
 // Lifetime: a request holds a reference to its stream, so a stream closed between the
 // completion of its substep and the wait for it stays valid until that wait returns.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -221,13 +222,19 @@ struct hpo_server {
       off += st.n;
     }
     HPO_HIP(hipMemcpyAsync(sl.d_dt, sl.h_dt, total * sizeof(double), hipMemcpyHostToDevice, sl.stream));
-    const int32_t* order = nullptr;
-    if (opts.opts.sort_cells == 1 && total > 1) {
-      HPO_HIP(hpo::launch_t0_order(sl.d_y, total, ld, sl.d_owork, sl.d_order, sl.stream));
-      order = sl.d_order;
+    // consecutive launches of at most launch_cells packed rows each (one when 0), each with its
+    // own work-queue order
+    const int64_t chunk = opts.launch_cells > 0 ? opts.launch_cells : total;
+    for (int64_t c0 = 0; c0 < total; c0 += chunk) {
+      const int64_t cnt = std::min(chunk, total - c0);
+      const int32_t* order = nullptr;
+      if (opts.opts.sort_cells == 1 && cnt > 1) {
+        HPO_HIP(hpo::launch_t0_order(sl.d_y + c0 * ld, cnt, ld, sl.d_owork, sl.d_order + c0, sl.stream));
+        order = sl.d_order + c0;
+      }
+      launch_integrate(m, cnt, sl.d_y + c0 * ld, sl.d_y + c0 * ld, ld, sl.d_P + c0, sl.d_dt + c0, sl.d_status + c0,
+                       nullptr, order, nullptr, opts.opts, sl.stream, nullptr, &sl.res);
     }
-    launch_integrate(m, total, sl.d_y, sl.d_y, ld, sl.d_P, sl.d_dt, sl.d_status, nullptr, order, nullptr,
-                     opts.opts, sl.stream, nullptr, &sl.res);
     for (SrvReq* r : batch) {
       SrvStream& st = *r->st;
       HPO_HIP(hipMemcpyAsync(st.d_y, sl.d_y + r->offset * ld, st.n * ld * sizeof(double), hipMemcpyDeviceToDevice,
@@ -371,6 +378,7 @@ int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_
     if (s->opts.n_slots <= 0) s->opts.n_slots = 8;
     if (s->opts.n_slots > 16) throw std::invalid_argument("n_slots must be at most 16");
     if (s->opts.max_batch_cells <= 0) s->opts.max_batch_cells = 262144;
+    if (s->opts.launch_cells < 0) throw std::invalid_argument("launch_cells must be 0 or positive");
     if (s->opts.cu_share_pct == 0) s->opts.cu_share_pct = 100;
     if (s->opts.cu_share_pct < 1 || s->opts.cu_share_pct > 100)
       throw std::invalid_argument("cu_share_pct must be in 0..100");

@@ -212,6 +212,9 @@ typedef struct {
   hpo_opts opts;             /* integration options for every launch; sort_cells = 1 (the
                                 default here) orders each launch's cells by descending initial
                                 temperature on the device, 0 takes them in packing order */
+  int64_t launch_cells;      /* a batch of more cells runs as consecutive launches of at most
+                                this many on its slot, so that other slots' launches reach the
+                                device between them; 0 = one launch per batch */
 } hpo_server_opts;
 typedef struct {
   double latency_ms;         /* submit -> results available to the caller */

@@ -185,12 +185,13 @@ def test_sixteen_slots_under_load(mechs):
 
 def test_device_order_changes_no_result(mechs):
     """Each launch's work-queue order is built on the device (hottest cells first, the default):
-    every cell is integrated once, to the same result as in packing order."""
+    every cell is integrated once, to the same result as in packing order, and as when the
+    batch runs as consecutive launches of at most 1,000 cells (launch_cells)."""
     gri = mechs[1]
     y0, P = _state(gri, 1, 5000, 41)
     out = []
-    for sort in (1, 0):
-        srv = H.Server(mechs, n_slots=2, sort_cells=sort)
+    for sort, lc in ((1, 0), (0, 0), (1, 1000)):
+        srv = H.Server(mechs, n_slots=2, sort_cells=sort, launch_cells=lc)
         try:
             sid = srv.open_stream(1, y0, P)
             y, s, _ = srv.wait(srv.submit(sid, 6e-5, n=5000, n_state=gri.n_state))
@@ -199,5 +200,5 @@ def test_device_order_changes_no_result(mechs):
             srv.close()
         assert (s == 0).all()
         out.append(y)
-    assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
     assert not np.array_equal(out[0][:, 0], y0[:, 0])
