@@ -675,8 +675,9 @@ HPO_DI double blk_gj_apply(const BlkInv<NS, F>& L, double b, const BlkLds& w, in
 // Each entry sees the same FMAs in the same order as in the step-by-step elimination, so the
 // result is the same bit for bit. On 512 ESTiMatE-Mech cells (214 variables) the blocks took
 // the FP64 kernel from 3,412 to 986 ms and the FP32 one from 3,264 to 1,165 ms, with identical
-// results; 4-step blocks in FP64 gave 1,353 ms and 16-step blocks in FP32 1,176 ms. step: as
-// BlkInv::step.
+// results; 4-step blocks in FP64 gave 1,353 ms and 16-step blocks in FP32 1,176 ms. Loading
+// the sweep's columns (and blk_gj_apply_g's) in groups ahead of their updates then gave 908 /
+// 1,073 ms. step: as BlkInv::step.
 template <int NS, typename F>
 HPO_DI bool blk_gj_invert_g(F* M, int& step, const BlkLds& w, int tid) {
   constexpr int B = kBlkGjSteps;
@@ -737,12 +738,20 @@ HPO_DI bool blk_gj_invert_g(F* M, int& step, const BlkLds& w, int tid) {
       l[s] = isp ? F(0) : akk * mult;
     }
     if (tid < NS) {
-      for (int j = 0; j < NS; ++j) {
-        F v = M[j * kBlkThreads + tid];
+      constexpr int G = 8;   // groups of columns: their loads ahead of their updates
+      for (int j0 = 0; j0 < NS; j0 += G) {
+        F v[G];
 #pragma unroll
-        for (int s2 = 0; s2 < B; ++s2)
-          if (s2 < nb) v = fma(-l[s2], prow[s2 * NS + j], v);
-        M[j * kBlkThreads + tid] = v;
+        for (int q = 0; q < G; ++q) v[q] = j0 + q < NS ? M[(j0 + q) * kBlkThreads + tid] : F(0);
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+#pragma unroll
+          for (int s2 = 0; s2 < B; ++s2)
+            if (s2 < nb && j0 + q < NS) v[q] = fma(-l[s2], prow[s2 * NS + j0 + q], v[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          if (j0 + q < NS) M[(j0 + q) * kBlkThreads + tid] = v[q];
       }
       // the block's own pivot columns (overwriting what the loop above stored there): the
       // value each row took at the column's step, then the block's later steps
@@ -772,12 +781,23 @@ HPO_DI double blk_gj_apply_g(const F* M, int step, double b, const BlkLds& w, in
   __syncthreads();
   F x0 = F(0), x1 = F(0);
   if (tid < NS) {
-    int j = 0;
-    for (; j + 1 < NS; j += 2) {
-      x0 = fma(M[j * kBlkThreads + tid], static_cast<F>(w.vec2[j]), x0);
-      x1 = fma(M[(j + 1) * kBlkThreads + tid], static_cast<F>(w.vec2[j + 1]), x1);
+    // groups of 16 column loads issued before their products (the same products in the same
+    // order as one pair at a time)
+    constexpr int G = 16;
+    for (int j0 = 0; j0 < NS; j0 += G) {
+      F m[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) m[q] = j0 + q < NS ? M[(j0 + q) * kBlkThreads + tid] : F(0);
+#pragma unroll
+      for (int q = 0; q < G; q += 2) {
+        if (j0 + q + 1 < NS) {
+          x0 = fma(m[q], static_cast<F>(w.vec2[j0 + q]), x0);
+          x1 = fma(m[q + 1], static_cast<F>(w.vec2[j0 + q + 1]), x1);
+        } else if (j0 + q < NS) {
+          x0 = fma(m[q], static_cast<F>(w.vec2[j0 + q]), x0);
+        }
+      }
     }
-    if (j < NS) x0 = fma(M[j * kBlkThreads + tid], static_cast<F>(w.vec2[j]), x0);
   }
   if (step >= 0) w.vec[step] = static_cast<double>(x0 + x1);
   __syncthreads();
