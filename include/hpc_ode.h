@@ -129,11 +129,14 @@ int hpo_batch_get(const hpo_batch *b, int64_t first, int64_t count, double *y, i
 /* d_order: work-queue order of the cells (a permutation of 0..ncells-1) or NULL.
  * d_prof: [ncells][HPO_NPROF] shader-clock cycles per phase (RHS, Jacobian, LU factorization,
  * LU solves, whole cell) or NULL. With timing != NULL the call waits for the kernel;
- * otherwise it returns after the launch. With timing == NULL the call enqueues two kernels on
- * hip_stream (a work-queue counter reset and the integration) and can be captured into a HIP
- * graph once an uncaptured call of at least as many cells has run on the same mechanism. The
- * graph uses the mechanism's counter and scratch: its replays must not overlap other calls on
- * that mechanism, and a later call with more cells (which grows the scratch) invalidates it.
+ * otherwise it returns after the launch. With timing == NULL the call enqueues its work on
+ * hip_stream: a work-queue counter reset, the integration and, with the FP64 fallback armed
+ * (lu_precision 1, fp64_fallback 1), a copy of the input rows first when the call runs in
+ * place, then a failed-cell count and the FP64 launch; kernels and one device copy only. It can
+ * be captured into a HIP graph once an uncaptured call of at least as many cells, in place or
+ * not alike, has run on the same mechanism. The graph uses the mechanism's counter, scratch and
+ * row copy: its replays must not overlap other calls on that mechanism, and a later call with
+ * more cells (which grows them) invalidates it.
  * Uncaptured calls on one mechanism may come from several host threads and streams: they share
  * the mechanism's counter and scratch, so each launch waits on the device for the previous one
  * (an event, when the stream differs) and they never overlap. Launches that must overlap need
