@@ -57,7 +57,7 @@ def parse():
                     help="serving: largest share of the device one launch may take, in percent (0 = 100)")
     ap.add_argument("--serve-loads", default="0.3,0.7",
                     help="offered loads as fractions of the measured serving capacity")
-    ap.add_argument("--serve-substeps", type=int, default=1000,
+    ap.add_argument("--serve-substeps", type=int, default=2000,
                     help="serving: each load runs until it has offered at least this many substeps")
     ap.add_argument("--serve-slots", type=int, default=8, help="serving: launch slots (1..16)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -80,23 +80,26 @@ def measured_traffic(workload, ncells, lu_precision):
     return rec.get("traffic_bytes")
 
 
-def serving_measurement(H, gri, seconds, fractions, cu_share=0, min_substeps=1000, slots=8):
+def serving_measurement(H, gri, seconds, fractions, cu_share=0, min_substeps=2000, slots=8):
     """p50/p99 substep latency of the serving loop at offered loads stated as fractions of its
     measured capacity (requests: 70 % H2/air, 30 % GRI-3.0, log-uniform [1, 65536] cells,
     dt U[1e-5, 1e-4] s, two substeps per request stream, Poisson arrivals), each load offered
-    for its share of `seconds` or until it has offered min_substeps substeps."""
+    for its share of `seconds` or until it has offered min_substeps substeps. The arrival rate
+    is set from the mean size of the requests the load will draw (its first min_substeps / 2),
+    so that the offered cells per second are the stated fraction of the capacity."""
     from hpc_ode_amd import serving as SV
     h2 = H.Mechanism(os.path.join(ROOT, "hpc_ode_amd", "mechanisms", "h2o2.yaml"))
     mechs = [h2, gri]
     srv = H.Server(mechs, n_slots=slots, cu_share_pct=cu_share)
+    seed, substeps = 11, 2
     try:
         cap = SV.capacity(srv, mechs)
-        mc = SV.mean_cells()
-        substeps = 2
+        drawn, _, _ = SV.request_mix(-(-min_substeps // substeps), seed + 1)
+        mc = float(drawn.mean())
         loads = []
         for f in fractions:
             rate = f * cap / (mc * substeps)
-            r = SV.run_load(srv, mechs, rate, seconds / len(fractions), seed=11, substeps=substeps,
+            r = SV.run_load(srv, mechs, rate, seconds / len(fractions), seed=seed, substeps=substeps,
                             log=lambda m: print(m, file=sys.stderr, flush=True), min_substeps=min_substeps)
             r["fraction_of_capacity"] = f
             loads.append(r)
