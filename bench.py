@@ -107,6 +107,7 @@ def serving_measurement(H, gri, seconds, fractions, cu_share=0, min_substeps=200
                 "cu_share_pct": cu_share or 100,
                 "mix": "70% H2/air (h2o2.yaml), 30% GRI-3.0; cells log-U[1,65536]; dt U[1e-5,1e-4] s; "
                        "2 substeps per stream with state resident between them; Poisson arrivals",
+                "precision": "the server's default: FP32 Newton matrix, one FP64 refinement sweep, FP64 fallback",
                 "loads": loads}
     finally:
         srv.close()

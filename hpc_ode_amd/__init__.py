@@ -335,7 +335,8 @@ class Server:
         so.cu_share_pct = cu_share_pct
         if launch_cells is not None:
             so.launch_cells = launch_cells
-        opt_kw.setdefault("sort_cells", 1)   # the server's default: hottest cells first in each launch
+        opt_kw.setdefault("sort_cells", 1)   # the server's defaults: hottest cells first in each launch,
+        opt_kw.setdefault("lu_precision", 1)  # FP32 Newton matrix with FP64 refinement and fallback
         so.opts = opts(**opt_kw)
         self.mechs = list(mechs)
         arr = (C.c_void_p * len(self.mechs))(*[m.handle.value for m in self.mechs])

@@ -366,6 +366,7 @@ void hpo_server_opts_default(hpo_server_opts* o) {
   o->max_batch_cells = 262144;
   hpo_opts_default(&o->opts);
   o->opts.sort_cells = 1;   // each launch pulls its hottest cells first (an order built on the device)
+  o->opts.lu_precision = 1; // FP32 Newton matrix with FP64 refinement (and the FP64 fallback)
 }
 
 int hpo_server_create(hpo_mech* const* mechs, int32_t n_mechs, const hpo_server_opts* o, hpo_server** out) {

@@ -212,9 +212,11 @@ typedef struct {
                                 arrives while a large one runs finds free slots at once. */
   int64_t max_batch_cells;   /* cells per launch; 0 = 262144 */
   int64_t hbm_budget_bytes;  /* device memory for resident streams; 0 = 90% of free HBM at creation */
-  hpo_opts opts;             /* integration options for every launch; sort_cells = 1 (the
-                                default here) orders each launch's cells by descending initial
-                                temperature on the device, 0 takes them in packing order */
+  hpo_opts opts;             /* integration options for every launch. Defaults here differ from
+                                hpo_opts_default in two fields: sort_cells = 1 orders each
+                                launch's cells by descending initial temperature on the device
+                                (0 takes them in packing order), and lu_precision = 1 (FP32
+                                Newton matrix, FP64 refinement, FP64 fallback) */
   int64_t launch_cells;      /* a batch of more cells runs as consecutive launches of at most
                                 this many on its slot, so that other slots' launches reach the
                                 device between them; 0 = one launch per batch */

@@ -116,6 +116,7 @@ def test_server_option_checks_come_before_device_calls(h2_path):
         so = H.ServerOpts()
         L.hpo_server_opts_default(C.byref(so))
         assert so.cu_share_pct == 0 and so.n_slots == 8 and so.opts.sort_cells == 1
+        assert so.opts.lu_precision == 1 and so.opts.fp64_fallback == 1
         setattr(so, field, bad)
         h = C.c_void_p()
         assert L.hpo_server_create(arr, 1, C.byref(so), C.byref(h)) == -1

@@ -49,8 +49,9 @@ def test_result_same_alone_and_in_mix(mechs, cu_share):
             srv.wait(r)
         assert ib["batch_requests"] > 1 or ib["batch_cells"] > 100
         assert np.array_equal(ya, yb) and np.array_equal(sa, sb)
-        # and the same as the one-shot integration entry point
-        yc, sc, _ = gri.integrate(y0, P, np.full(100, 5e-5))
+        # and the same as the one-shot integration entry point with the server's default
+        # options (FP32 Newton matrix with the FP64 fallback)
+        yc, sc, _ = gri.integrate(y0, P, np.full(100, 5e-5), lu_precision=1)
         assert np.array_equal(ya, yc)
     finally:
         srv.close()
@@ -66,8 +67,8 @@ def test_substeps_continue_from_resident_state(mechs):
         y2, s2, _ = srv.wait(srv.submit(sid, 6e-5, n=257, n_state=h2.n_state))
         assert np.array_equal(srv.stream_state(sid, 257, h2.n_state), y2)
         srv.close_stream(sid)
-        y1, _, _ = h2.integrate(y0, P, np.full(257, 4e-5))
-        yr, _, _ = h2.integrate(y1, P, np.full(257, 6e-5))
+        y1, _, _ = h2.integrate(y0, P, np.full(257, 4e-5), lu_precision=1)
+        yr, _, _ = h2.integrate(y1, P, np.full(257, 6e-5), lu_precision=1)
         assert np.array_equal(y2, yr)
     finally:
         srv.close()
