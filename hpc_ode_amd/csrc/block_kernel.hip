@@ -649,15 +649,19 @@ HPO_DI bool blk_gj_invert(BlkInv<NS, F>& L, const BlkLds& w, int tid) {
 template <int NS, typename F>
 HPO_DI double blk_gj_apply(const BlkInv<NS, F>& L, double b, const BlkLds& w, int tid) {
   __syncthreads();   // previous readers of vec / vec2 are done
-  if (L.step >= 0) w.vec2[L.step] = b;
+  // b in the matrix's precision, converted once by the thread that stores it rather than by
+  // every reader (nDodecane FP32 -17 %, GRI block -7 %, same results; with the matrix in
+  // scratch the same change made ESTiMatE 16 % slower, so blk_gj_apply_g keeps doubles)
+  F* bv = reinterpret_cast<F*>(w.vec2);
+  if (L.step >= 0) bv[L.step] = static_cast<F>(b);
   __syncthreads();
   F x0 = F(0), x1 = F(0);
 #pragma unroll
   for (int j = 0; j + 1 < NS; j += 2) {
-    x0 = fma(L.a[j], static_cast<F>(w.vec2[j]), x0);
-    x1 = fma(L.a[j + 1], static_cast<F>(w.vec2[j + 1]), x1);
+    x0 = fma(L.a[j], bv[j], x0);
+    x1 = fma(L.a[j + 1], bv[j + 1], x1);
   }
-  if (NS & 1) x0 = fma(L.a[NS - 1], static_cast<F>(w.vec2[NS - 1]), x0);
+  if (NS & 1) x0 = fma(L.a[NS - 1], bv[NS - 1], x0);
   if (L.step >= 0) w.vec[L.step] = static_cast<double>(x0 + x1);
   __syncthreads();
   return tid < NS ? w.vec[tid] : 0.0;
